@@ -1,0 +1,205 @@
+"""Benchmark of the single-curve modular square sieve on one MI355X.
+
+Workload: the planted genus-2 sextic of seed 2 (q coefficients in
+[-2^30, 2^30], six planted points with |a_i|, b_i <= 5e6, ~150-bit
+coefficients), box a in [-5e6, 5e6], b in [1, 5e6], denominator batches of
+65,536 -- the plain method at full size.
+
+A step is one denominator batch: 65,536 lines of 2A + 1 = 10,000,001 sites
+(the last batch of the box has 19,264 lines).  The default run times the
+whole box (77 steps, 5.00000005e13 sites) after 2 untimed warmup batches, in
+one rps_sieve call so the survivor stream overlaps the kernels as in
+production.  Inputs (the curve's residue table) are resident before timing.
+
+Printed: one JSON line.  ``value`` is Gsite/s over the timed batches
+(kernels + survivor download + sink); ``roofline`` uses the sieve kernel's own
+HIP-event time; ``cpu_baseline`` times the CPU oracle's packed-word sieve on a
+bounded sample of the same workload on the host cores.
+"""
+
+import argparse
+import importlib.util
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+BATCH = 65536
+BOUND = 5_000_000
+LAST = 5_000_000
+# LDS bytes the dense stage reads per site: 14 primes x one 64-bit row word
+# per 64 numerators (the reference's fast stage reads 14 x 32-bit words per
+# 32 numerators, the same 1.75 B/site).
+LDS_BYTES_PER_SITE = 14 * 8 / 64
+# MI355X LDS read bandwidth: 256 CUs x 128 B/clk x 2.4 GHz.
+LDS_PEAK_GBS = 256 * 128 * 2.4
+RTX2000_ADA_GSITES = 879.5   # published, A = B = 5e6 (README.md:40-47)
+RTX4090_GSITES = 7423.0      # published, A = B = 1e7 end to end
+
+
+def load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    module = importlib.util.module_from_spec(spec)
+    sys.modules[name] = module
+    spec.loader.exec_module(module)
+    return module
+
+
+def cpu_baseline(coeffs, seconds_target=15.0):
+    """The oracle's packed-word CPU sieve on the same curve and A, over the
+    first n denominators, n sized for about `seconds_target` of wall time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    lib = oracle.OracleLib()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    n = threads
+    t0 = time.perf_counter()
+    lib.sieve(coeffs, BOUND, 1, n, method="words", threads=threads, collect=False)
+    dt = time.perf_counter() - t0
+    n = max(threads, int(n * seconds_target / max(dt, 1e-3)))
+    n = min(n, LAST)
+    t0 = time.perf_counter()
+    count, _, _, _ = lib.sieve(coeffs, BOUND, 1, n, method="words", threads=threads,
+                               collect=False)
+    dt = time.perf_counter() - t0
+    sites = (2 * BOUND + 1) * n
+    return {
+        "value": round(sites / dt / 1e9, 3),
+        "unit": "Gsite/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle packed-word sieve, same curve, A=5e6, b=1..{n} "
+                  f"({sites:.3e} sites, {dt:.1f} s, {count} survivors)",
+    }
+
+
+def verify_points(coeffs, nums, dens):
+    """Exact stage on the survivors (gcd = 1, F(a, b) a perfect square)."""
+    n = len(coeffs) - 1
+    points = []
+    for a, b in zip(nums, dens):
+        a, b = int(a), int(b)
+        if math.gcd(a, b) != 1:
+            continue
+        v = sum(c * a ** i * b ** (n - i) for i, c in enumerate(coeffs))
+        if n % 2:
+            v *= b
+        if v >= 0 and math.isqrt(v) ** 2 == v:
+            points.append((a, b))
+    return points
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=77)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; one MI355X covers the whole "
+              "search box (denominator sharding is a host-side design note).", file=sys.stderr)
+        sys.exit(2)
+    pkg = load("ratpoints_gpu_amd", os.path.join(ROOT, "ratpoints-gpu_amd", "__init__.py"))
+    wl = load("ratpoints_gpu_amd_workloads", os.path.join(ROOT, "ratpoints-gpu_amd", "workloads.py"))
+    coeffs, planted = wl.config2_curve()
+    scoeffs = [str(c) for c in coeffs]
+    total_batches = (LAST + BATCH - 1) // BATCH
+    steps = max(1, min(args.steps, total_batches))
+    last_t = min(LAST, steps * BATCH)
+
+    ctx = pkg.SieveContext(0)
+    ctx.set_option("variant", args.variant)
+    ctx.set_option("batch_size", BATCH)
+    variant = pkg.variants()[args.variant]
+
+    # Warmup: the first `warmup` batches (also builds the residue table).
+    warm_last = min(LAST, max(1, args.warmup) * BATCH)
+    print(f"bench: warmup b=1..{warm_last} variant={variant}", file=sys.stderr, flush=True)
+    ctx.run(scoeffs, BOUND, 1, warm_last, False, lambda n, d: False)
+
+    nums, dens = [], []
+
+    def sink(num, den):
+        nums.append(num.copy())
+        dens.append(den.copy())
+        return False
+
+    print(f"bench: timed b=1..{last_t} ({steps} steps)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    rc, stats = ctx.run(scoeffs, BOUND, 1, last_t, False, sink)
+    wall = time.perf_counter() - t0
+    kernel_ms, launches, kernel_sites = ctx.kernel_timing()
+    ctx.close()
+
+    import numpy as np
+    num = np.concatenate(nums) if nums else np.zeros(0, dtype=np.int64)
+    den = np.concatenate(dens) if dens else np.zeros(0, dtype=np.int32)
+    t1 = time.perf_counter()
+    points = verify_points(coeffs, num.tolist(), den.tolist())
+    verify_s = time.perf_counter() - t1
+    planted_in_box = [p for p in planted if p[1] <= last_t]
+    missing = [p for p in planted_in_box if p not in points]
+
+    sites = (2 * BOUND + 1) * last_t
+    gsites = sites / wall / 1e9
+    kernel_gsites = kernel_sites / (kernel_ms / 1e3) / 1e9 if kernel_ms else 0.0
+    per_launch_ms = kernel_ms / launches if launches else 0.0
+    lds_bytes_per_launch = LDS_BYTES_PER_SITE * kernel_sites / max(launches, 1)
+    achieved = lds_bytes_per_launch / (per_launch_ms / 1e3) / 1e9 if per_launch_ms else 0.0
+
+    result = {
+        "metric": "Gsite/s single curve A=5e6",
+        "value": round(gsites, 2),
+        "unit": "Gsite/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(gsites / RTX2000_ADA_GSITES, 3),
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": "planted-sextic seed 2, A=5e6, b=1..%d, batch 65536" % last_t,
+            "bound": BOUND,
+            "denominators": last_t,
+            "batch": BATCH,
+            "variant": variant,
+            "sites": sites,
+        },
+        "roofline": {
+            "bound": "lds",
+            "achieved": round(achieved, 1),
+            "peak": LDS_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / LDS_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": variant,
+            "kernel_ms_per_launch": round(per_launch_ms, 3),
+            "bytes_per_site": LDS_BYTES_PER_SITE,
+        },
+        "kernel_gsites": round(kernel_gsites, 2),
+        "survivors": int(stats["survivor_count"]),
+        "points": len(points),
+        "planted_found": len(planted_in_box) - len(missing),
+        "planted_in_box": len(planted_in_box),
+        "verify_s": round(verify_s, 3),
+        "vs_rtx4090_record": round(gsites / RTX4090_GSITES, 3),
+    }
+    if missing:
+        result["error"] = f"planted points missing: {missing}"
+    if not args.no_cpu:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(coeffs)
+    print(json.dumps(result), flush=True)
+    if missing:
+        sys.exit(1)
+
+
+if __name__ == "__main__":
+    main()

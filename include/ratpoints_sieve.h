@@ -1,0 +1,147 @@
+/*
+ * ratpoints_sieve.h -- C ABI of the MI355X (gfx950) modular square sieve.
+ *
+ * This is the drop-in boundary for the search backend of ratpoints_gpu.  It
+ * replaces, one for one, what the reference binds at its backend interface
+ * (paths relative to the reference repository):
+ *
+ *   run_modular_sieve(coefficients, numerator_bound, denominators, callback,
+ *                     square_denominators)             include/sieve.hpp:43-47
+ *   CandidateBatch / CandidateBatchCallback            include/sieve.hpp:32-40
+ *   SieveResult / SieveMetrics                         include/sieve.hpp:13-30
+ *   check_cuda error convention (op: message)          include/cuda_support.hpp:14-19
+ *   available_gpu_devices / activate_gpu               include/gpu_devices.hpp:20-23
+ *
+ * Only plain C types cross it: pointers, sizes and fixed-width integers.  The
+ * C++ surface with the reference's exact signature (ratpoints_gpu::
+ * run_modular_sieve) is a thin shim over these functions, in
+ * ratpoints-gpu_amd/host/sieve_shim.cpp.
+ *
+ * Threading: one rps_ctx per thread; distinct contexts may run concurrently.
+ * A context binds its device at rps_open and owns its streams, device tables
+ * and pinned staging buffers until rps_close.
+ */
+#ifndef RATPOINTS_SIEVE_H
+#define RATPOINTS_SIEVE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RPS_ABI_VERSION 1
+
+/* Status codes.  Every entry point returning int returns one of these; the
+ * message of the last failure on a context is rps_last_error(ctx). */
+enum {
+    RPS_OK = 0,
+    RPS_EINVAL = 1,     /* invalid argument (std::invalid_argument)        */
+    RPS_EDEVICE = 2,    /* HIP runtime or kernel failure (runtime_error)   */
+    RPS_ENOMEM = 3,     /* device or pinned allocation failed              */
+    RPS_ECANCELLED = 4, /* the sink asked to stop; not an error for -1     */
+    RPS_EOVERFLOW = 5,  /* survivor total overflowed (overflow_error)      */
+    RPS_EINTERNAL = 6   /* invariant violated (logic_error)                */
+};
+
+typedef struct rps_ctx rps_ctx;
+
+/* One sieve request: the reference's run_modular_sieve arguments
+ * (include/sieve.hpp:43-47, include/search_types.hpp:9-22). */
+typedef struct rps_request {
+    const char *const *coefficients; /* decimal c0 .. cn, optional sign     */
+    int32_t n_coefficients;          /* n + 1, 2..101                       */
+    int64_t numerator_bound;         /* A >= 1: numerators a in [-A, A]     */
+    int32_t den_first;               /* first denominator parameter, >= 1   */
+    int32_t den_last;                /* last parameter, inclusive           */
+    int32_t square_denominators;     /* nonzero: b = t^2, den_last <= 46340 */
+} rps_request;
+
+/* Survivor sink: the reference's CandidateBatchCallback
+ * (include/sieve.hpp:38-40).  `numerators[i] / denominators[i]` are raw
+ * modular survivors (a, b); b is t^2 in square mode.  Order within and across
+ * calls is unspecified, each survivor is delivered exactly once.  The arrays
+ * are valid only during the call.  Return 0 to continue; any nonzero value
+ * cancels the request, which then returns RPS_ECANCELLED after the current
+ * denominator batch. */
+typedef int (*rps_sink)(void *user, const int64_t *numerators,
+                        const int32_t *denominators, size_t count);
+
+/* Metrics of one request: SieveResult + SieveMetrics
+ * (include/sieve.hpp:13-30).  word_count and initial_mask_bytes keep the
+ * reference's definitions (32-bit words, (2A+32)/32; den x words x 14 x 4 B)
+ * so the -v metrics line stays comparable. */
+typedef struct rps_stats {
+    uint64_t survivor_count;  /* raw modular survivors delivered          */
+    uint64_t word_count;      /* (2A + 32) / 32, as SievePlan::word_count */
+    double basis_ms;          /* device time of the per-curve table build */
+    double sieve_ms;          /* device time of all sieve launches        */
+    double plan_ms;
+    double workspace_ms;
+    double basis_wall_ms;
+    double survivor_setup_ms;
+    double verification_ms;   /* wall time spent inside the sink          */
+    double sieve_wall_ms;
+    double initial_mask_bytes;
+    uint64_t launches;        /* sieve kernel launches, resumes included  */
+    uint64_t batches;         /* denominator batches                      */
+} rps_stats;
+
+/* Opens a context on HIP device `device` (0-based among visible devices).
+ * Replaces activate_gpu (include/gpu_devices.hpp:23) plus the per-call
+ * workspace construction of src/sieve.cu:65-95, hoisted to open time. */
+int rps_open(int device, rps_ctx **out);
+
+/* Releases every device and host resource of the context. */
+void rps_close(rps_ctx *ctx);
+
+/* Message of the last failure on `ctx` ("" if none).  A NULL ctx returns
+ * the message of the last failed rps_open on this thread. */
+const char *rps_last_error(const rps_ctx *ctx);
+
+/* Runs the modular square sieve over [-A, A] x [den_first, den_last] and
+ * streams every survivor to `sink`.  Replaces run_modular_sieve
+ * (include/sieve.hpp:43-47, src/sieve.cu:503-580).  Errors follow the
+ * reference: invalid range -> RPS_EINVAL ("invalid denominator range",
+ * src/sieve_plan.cpp:55-58), HIP failures -> RPS_EDEVICE ("<op>: <msg>").
+ * `stats` may be NULL. */
+int rps_sieve(rps_ctx *ctx, const rps_request *request, rps_sink sink,
+              void *user, rps_stats *stats);
+
+/* Tuning and test knobs; returns RPS_EINVAL for an unknown name or a value
+ * out of range.  No counterpart in the reference, whose knobs were
+ * compile-time macros (src/sieve_plan.hpp:10-22, src/sieve.cu:15-17):
+ *   "survivor_capacity"  survivors per flush buffer, >= 16384 (default 2^24)
+ *   "batch_size"         denominators per launch, 1..1048576 (default 65536)
+ *   "variant"            kernel variant index, 0..rps_variant_count()-1
+ *   "min_tiles"          split denominators into word chunks until at least
+ *                        this many tiles per launch exist (default 2048)
+ */
+int rps_set_option(rps_ctx *ctx, const char *name, int64_t value);
+
+/* Number of compiled kernel variants and a short name for each, e.g.
+ * "u2_f14_w8" (words per lane, dense primes, waves per workgroup). */
+int rps_variant_count(void);
+const char *rps_variant_name(int variant);
+
+/* Device enumeration for --list-devices (include/gpu_devices.hpp:20-21,
+ * src/gpu_devices.cpp:55-97).  rps_device_info writes a NUL-terminated name
+ * and PCI bus id; any output pointer may be NULL. */
+int rps_device_count(int *count);
+int rps_device_info(int device, char *name, size_t name_len, char *pci,
+                    size_t pci_len, int *compute_units,
+                    uint64_t *memory_bytes, int *major, int *minor);
+
+/* Timing of the dominant kernel over the last rps_sieve call on `ctx`,
+ * measured with HIP events on the stream the kernel runs on: total device
+ * milliseconds of all sieve launches, launch count, and the number of
+ * numerator sites those launches covered ((2A+1) x denominators). */
+int rps_last_kernel_timing(const rps_ctx *ctx, double *kernel_ms,
+                           uint64_t *launches, double *sites);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RATPOINTS_SIEVE_H */

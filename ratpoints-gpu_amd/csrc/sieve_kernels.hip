@@ -1,0 +1,442 @@
+// gfx950 kernels of the modular square sieve.
+//
+//   chi_kernel     chi_p bitmaps of one curve (src/sieve_plan.cpp:115-138 is
+//                  the host routine it replaces)
+//   resid_kernel   per (p, b mod p) residue strings; replaces the reference's
+//                  p x p 32-bit basis table (basis_kernel, src/sieve.cu:141-172)
+//                  with a 72-byte string per (p, b mod p) that the sieve
+//                  kernel expands into rows in LDS
+//   sieve_kernel   the hot loop (src/sieve.cu:221-388 is the kernel it
+//                  replaces): one workgroup per (denominator, word chunk)
+//                  tile; dense primes ANDed from LDS rows with wave-uniform
+//                  offsets; remaining primes applied to surviving words only;
+//                  survivors compacted with a wave prefix sum and one atomic
+//                  reservation per wave, with a resumable per-wave cursor.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "sieve_layout.hpp"
+#include "sieve_kernels.hpp"
+
+namespace rps {
+namespace {
+
+// x mod p for x < 2^32 with magic = floor(2^32 / p): the quotient estimate is
+// at most one too small, so one conditional subtraction finishes.
+__device__ __forceinline__ uint32_t mod_small(uint32_t x, uint32_t p,
+                                              uint32_t magic) {
+    uint32_t q = __umulhi(x, magic);
+    uint32_t r = x - q * p;
+    return r >= p ? r - p : r;
+}
+
+// 64 bits of the residue string `c` starting at bit o (o + 63 < 576).
+__device__ __forceinline__ uint64_t window64(const uint64_t *c, uint32_t o) {
+    uint32_t q = o >> 6;
+    uint32_t sh = o & 63;
+    uint64_t lo = c[q];
+    uint64_t hi = c[q + 1];
+    return sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) {
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+}  // namespace
+
+// One workgroup of 512 threads per candidate prime.  Thread x evaluates
+// f(x) mod p by Horner's rule on the coefficient residues and tests it
+// against the table of squares mod p (zero counts as a square).
+__global__ __launch_bounds__(512) void chi_kernel(
+    const int *__restrict__ coef_res,  // [36][n_coef]
+    int n_coef, const int *__restrict__ primes,
+    unsigned long long *__restrict__ chi) {  // [36][8]
+    __shared__ unsigned char square[kMaxPrime];
+    __shared__ int res[kMaxCoefficients];
+    const int i = blockIdx.x;
+    const int x = threadIdx.x;
+    const int p = primes[i];
+    square[x] = 0;
+    for (int k = x; k < n_coef; k += blockDim.x) {
+        res[k] = coef_res[i * n_coef + k];
+    }
+    __syncthreads();
+    if (x < p) {
+        square[(x * x) % p] = 1;
+    }
+    __syncthreads();
+    int bit = 0;
+    if (x < p) {
+        int v = 0;
+        for (int k = n_coef - 1; k >= 0; --k) {
+            v = (v * x + res[k]) % p;
+        }
+        bit = square[v];
+    }
+    unsigned long long word = __ballot(bit);
+    if ((x & 63) == 0) {
+        chi[i * kChiWords + (x >> 6)] = word;
+    }
+}
+
+// Residue strings: one wave per (prime index i, u).  For u = 0 (p | b) the
+// string is never read and is stored as zero.  Bit z of the string is
+// chi_p((z mod p) * u^-1 mod p); ballot packs 64 consecutive z per word.
+__global__ __launch_bounds__(256) void resid_kernel(
+    const int *__restrict__ primes,
+    const unsigned long long *__restrict__ chi,
+    unsigned long long *__restrict__ resid) {  // [36][512][9]
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int u = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const uint32_t p = primes[i];
+    unsigned long long *dst =
+        resid + (static_cast<size_t>(i) * kMaxPrime + u) * kResidWords;
+    if (u == 0 || u >= static_cast<int>(p)) {
+        if (lane < kResidWords) {
+            dst[lane] = 0;
+        }
+        return;
+    }
+    // u^-1 = u^(p-2) mod p (Fermat; p is prime).
+    uint32_t inv = 1, base = u, e = p - 2;
+    while (e) {
+        if (e & 1) {
+            inv = inv * base % p;
+        }
+        base = base * base % p;
+        e >>= 1;
+    }
+    unsigned long long mine = 0;
+    for (int k = 0; k < kResidWords; ++k) {
+        uint32_t z = 64 * k + lane;
+        if (z >= p) {
+            z -= p;  // z < 576 < 2p
+        }
+        uint32_t y = z * inv % p;
+        int bit = static_cast<int>((chi[i * kChiWords + (y >> 6)] >> (y & 63)) & 1);
+        unsigned long long word = __ballot(bit);
+        if (lane == k) {
+            mine = word;
+        }
+    }
+    if (lane < kResidWords) {
+        dst[lane] = mine;
+    }
+}
+
+__global__ void reset_state_kernel(DeviceState *state) {
+    state->count = 0;
+    state->first_fail = ~0ULL;
+    state->full = 0;
+    state->pad = 0;
+}
+
+// The sieve.  Template parameters:
+//   U      64-bit words per lane per iteration (a wave covers 64 U words)
+//   FAST   selected primes ANDed densely for every word; the other
+//          32 - FAST are applied only to words with bits left
+//   WAVES  waves per workgroup; each wave sweeps a contiguous word range
+template <int U, int FAST, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void sieve_kernel(
+    const SieveParams P) {
+    constexpr int kThreads = WAVES * 64;
+    constexpr int kSlot = kMaxPrime + 64 * U;  // >= p + 64 U - 1 words
+    constexpr int kStride = 64 * U;
+    static_assert(kStride < 293, "a row step must stay below every prime");
+    static_assert(FAST >= 1 && FAST <= kSelected, "dense prime count");
+
+    __shared__ uint64_t rows[FAST * kSlot];
+    __shared__ uint64_t cext[kSelected * 10];
+    __shared__ uint32_t s_p[kSelected];
+    __shared__ uint32_t s_c[kSelected];
+    __shared__ uint32_t s_u[kSelected];
+    __shared__ uint32_t s_idx[kSelected];
+    __shared__ uint32_t s_base[kSelected];
+    __shared__ uint32_t s_magic[kSelected];
+    __shared__ int s_live;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = static_cast<int>(uniform(tid >> 6));
+    const int tile = blockIdx.x;
+    const int t_index = tile / P.tiles_per_t;
+    const int chunk = tile - t_index * P.tiles_per_t;
+    const long long t = static_cast<long long>(P.first_t) + t_index;
+    const int b = static_cast<int>(P.square ? t * t : t);
+
+    // Word range of this tile and of this wave (multiples of the stride).
+    const uint64_t ts = static_cast<uint64_t>(chunk) * P.tile_words;
+    const uint64_t te = ts + P.tile_words < P.words ? ts + P.tile_words : P.words;
+    const uint64_t per =
+        ((te - ts + static_cast<uint64_t>(kStride) * WAVES - 1)
+         / (static_cast<uint64_t>(kStride) * WAVES)) * kStride;
+    uint64_t ws = ts + per * wave;
+    const uint64_t we = ws + per < te ? ws + per : te;
+    if (ws > we) {
+        ws = we;
+    }
+    unsigned long long *cursor =
+        P.cursors + static_cast<size_t>(tile) * WAVES + wave;
+    if (P.resume) {
+        ws = *cursor;
+    }
+
+    // Skip the whole tile when every wave has already finished it.
+    if (tid == 0) {
+        s_live = 0;
+    }
+    __syncthreads();
+    if (lane == 0 && ws < we) {
+        atomicOr(&s_live, 1);
+    }
+    __syncthreads();
+    if (!s_live) {
+        return;
+    }
+
+    // Prime selection (src/sieve_plan.cpp:75-86): the 32 largest candidates
+    // that do not divide b.  Lane i tests candidate i; its slot is the number
+    // of qualifying candidates above it.
+    if (wave == 0) {
+        const bool cand = lane < kCandidates;
+        const uint32_t p = cand ? static_cast<uint32_t>(P.primes[lane]) : 1u;
+        const uint32_t u = cand ? static_cast<uint32_t>(b) % p : 0u;
+        const bool ok = cand && u != 0;
+        const unsigned long long mask = __ballot(ok);
+        if (ok) {
+            const int rank = __popcll(mask >> (lane + 1));
+            if (rank < kSelected) {
+                s_p[rank] = p;
+                s_c[rank] = static_cast<uint32_t>(P.neg_bound_mod[lane]);
+                s_u[rank] = u;
+                s_idx[rank] = lane;
+                s_magic[rank] = static_cast<uint32_t>(0x100000000ULL / p);
+                s_base[rank] = static_cast<uint32_t>(ts % p);
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < kSelected * kResidWords; k += kThreads) {
+        const int s = k / kResidWords;
+        const int w = k - s * kResidWords;
+        cext[s * 10 + w] = P.resid[(static_cast<size_t>(s_idx[s]) * kMaxPrime
+                                    + s_u[s]) * kResidWords + w];
+    }
+    __syncthreads();
+    // Dense rows: row s entry e is word e of the line mod p, i.e. the window
+    // of the residue string at (-A + 64 e) mod p; entries e >= p repeat the
+    // row so that reads never wrap.
+    for (int s = 0; s < FAST; ++s) {
+        const uint32_t p = s_p[s];
+        const uint32_t c = s_c[s];
+        const uint32_t magic = s_magic[s];
+        const uint64_t *cx = cext + s * 10;
+        const int n = static_cast<int>(p) + kStride - 1;
+        for (int e = tid; e < n; e += kThreads) {
+            const uint32_t o = mod_small(c + 64u * static_cast<uint32_t>(e), p, magic);
+            rows[s * kSlot + e] = window64(cx, o);
+        }
+    }
+    __syncthreads();
+
+    if (ws >= we) {
+        if (lane == 0) {
+            *cursor = we;
+        }
+        return;
+    }
+
+    // Wave-uniform row offsets: word ws mod p for each dense prime.
+    uint32_t off[FAST];
+    uint32_t prime[FAST];
+    {
+        const uint32_t rel = static_cast<uint32_t>(ws - ts);
+        uint32_t mine = 0;
+        if (lane < FAST) {
+            const uint32_t p = s_p[lane];
+            uint32_t v = mod_small(rel, p, s_magic[lane]) + s_base[lane];
+            mine = v >= p ? v - p : v;
+        }
+#pragma unroll
+        for (int s = 0; s < FAST; ++s) {
+            off[s] = __builtin_amdgcn_readlane(mine, s);
+            prime[s] = uniform(s_p[s]);
+        }
+    }
+
+    const uint64_t last_word = P.words - 1;
+    const uint64_t neg_a = static_cast<uint64_t>(-P.bound);
+    bool stopped = false;
+    uint64_t W = ws;
+    for (; W < we; W += kStride) {
+        uint64_t m[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            m[u] = ~0ULL;
+        }
+#pragma unroll
+        for (int s = 0; s < FAST; ++s) {
+            const uint64_t *r = rows + s * kSlot + off[s] + lane;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                m[u] &= r[64 * u];
+            }
+            const uint32_t next = off[s] + kStride;
+            off[s] = next >= prime[s] ? next - prime[s] : next;
+        }
+        if (W + kStride >= we) {
+            // Last iteration of this wave: drop words past the wave's range
+            // and the sites past +A in the line's final word.
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t w = W + lane + 64 * u;
+                if (w >= we) {
+                    m[u] = 0;
+                } else if (w == last_word) {
+                    m[u] &= P.last_mask;
+                }
+            }
+        }
+        uint64_t any = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            any |= m[u];
+        }
+        if (!__ballot(any != 0)) {
+            continue;
+        }
+        // Remaining primes, only for words that still have bits set.
+        const uint32_t rel = static_cast<uint32_t>(W - ts) + lane;
+        for (int s = FAST; s < kSelected; ++s) {
+            if (!__ballot(any != 0)) {
+                break;
+            }
+            if (any) {
+                const uint32_t p = s_p[s];
+                const uint32_t magic = s_magic[s];
+                const uint32_t base = s_base[s];
+                const uint32_t c = s_c[s];
+                const uint64_t *cx = cext + s * 10;
+                any = 0;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (m[u]) {
+                        uint32_t wm = mod_small(rel + 64 * u, p, magic) + base;
+                        wm = wm >= p ? wm - p : wm;
+                        const uint32_t o = mod_small(c + 64 * wm, p, magic);
+                        m[u] &= window64(cx, o);
+                    }
+                    any |= m[u];
+                }
+            }
+        }
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            cnt += __popcll(m[u]);
+        }
+        if (!__ballot(cnt != 0)) {
+            continue;
+        }
+        // Wave prefix sum of survivor counts, one reservation per wave.  The
+        // iteration is committed whole or not at all, so a failed wave can
+        // resume at W after the host has drained the buffer.
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t v = __shfl_up(incl, d, 64);
+            if (lane >= d) {
+                incl += v;
+            }
+        }
+        const uint32_t total = __shfl(incl, 63, 64);
+        unsigned long long start = 0;
+        if (lane == 0) {
+            start = atomicAdd(&P.state->count,
+                              static_cast<unsigned long long>(total));
+        }
+        start = __shfl(start, 0, 64);
+        if (start + total > P.capacity) {
+            if (lane == 0) {
+                atomicMin(&P.state->first_fail, start);
+                atomicOr(&P.state->full, 1u);
+            }
+            stopped = true;
+            break;
+        }
+        unsigned long long pos = start + incl - cnt;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            uint64_t mm = m[u];
+            const uint64_t a0 = neg_a + 64 * (W + lane + 64 * u);
+            while (mm) {
+                const int j = __ffsll(static_cast<long long>(mm)) - 1;
+                mm &= mm - 1;
+                P.out_num[pos] = static_cast<long long>(a0 + j);
+                P.out_den[pos] = b;
+                ++pos;
+            }
+        }
+    }
+    if (lane == 0) {
+        *cursor = stopped ? W : we;
+    }
+}
+
+namespace {
+
+template <int U, int FAST, int WAVES>
+hipError_t launch_variant(const SieveParams &params, unsigned grid,
+                          hipStream_t stream) {
+    hipLaunchKernelGGL((sieve_kernel<U, FAST, WAVES>), dim3(grid),
+                       dim3(WAVES * 64), 0, stream, params);
+    return hipGetLastError();
+}
+
+const VariantInfo kVariants[] = {
+    {"u2_f14_w8", 2, 14, 8, launch_variant<2, 14, 8>},
+    {"u1_f14_w8", 1, 14, 8, launch_variant<1, 14, 8>},
+    {"u4_f14_w8", 4, 14, 8, launch_variant<4, 14, 8>},
+    {"u2_f12_w8", 2, 12, 8, launch_variant<2, 12, 8>},
+    {"u2_f16_w8", 2, 16, 8, launch_variant<2, 16, 8>},
+    {"u2_f14_w16", 2, 14, 16, launch_variant<2, 14, 16>},
+    {"u4_f14_w16", 4, 14, 16, launch_variant<4, 14, 16>},
+    {"u4_f16_w16", 4, 16, 16, launch_variant<4, 16, 16>},
+};
+
+}  // namespace
+
+int variant_count() {
+    return static_cast<int>(sizeof(kVariants) / sizeof(kVariants[0]));
+}
+
+const VariantInfo *variant(int index) {
+    if (index < 0 || index >= variant_count()) {
+        return nullptr;
+    }
+    return &kVariants[index];
+}
+
+hipError_t launch_chi(const int *coef_res, int n_coef, const int *primes,
+                      unsigned long long *chi, hipStream_t stream) {
+    hipLaunchKernelGGL(chi_kernel, dim3(kCandidates), dim3(512), 0, stream,
+                       coef_res, n_coef, primes, chi);
+    return hipGetLastError();
+}
+
+hipError_t launch_resid(const int *primes, const unsigned long long *chi,
+                        unsigned long long *resid, hipStream_t stream) {
+    hipLaunchKernelGGL(resid_kernel, dim3(kCandidates, kMaxPrime / 4),
+                       dim3(256), 0, stream, primes, chi, resid);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset(DeviceState *state, hipStream_t stream) {
+    hipLaunchKernelGGL(reset_state_kernel, dim3(1), dim3(1), 0, stream, state);
+    return hipGetLastError();
+}
+
+}  // namespace rps

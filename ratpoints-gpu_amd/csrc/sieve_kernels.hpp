@@ -1,0 +1,32 @@
+// Host-side declarations of the gfx950 sieve kernels' launchers.
+#ifndef RATPOINTS_GPU_AMD_SIEVE_KERNELS_HPP
+#define RATPOINTS_GPU_AMD_SIEVE_KERNELS_HPP
+
+#include <hip/hip_runtime_api.h>
+
+#include "sieve_layout.hpp"
+
+namespace rps {
+
+// A compiled instantiation of the sieve kernel.
+struct VariantInfo {
+    const char *name;
+    int words_per_lane;  // U: a wave covers 64 U words per iteration
+    int fast_primes;     // primes ANDed for every word
+    int waves;           // waves per workgroup
+    hipError_t (*launch)(const SieveParams &params, unsigned grid,
+                         hipStream_t stream);
+};
+
+int variant_count();
+const VariantInfo *variant(int index);
+
+hipError_t launch_chi(const int *coef_res, int n_coef, const int *primes,
+                      unsigned long long *chi, hipStream_t stream);
+hipError_t launch_resid(const int *primes, const unsigned long long *chi,
+                        unsigned long long *resid, hipStream_t stream);
+hipError_t launch_reset(DeviceState *state, hipStream_t stream);
+
+}  // namespace rps
+
+#endif

@@ -1,0 +1,132 @@
+"""Seeded synthetic workloads for the sieve benchmarks and parity tests.
+
+Curves follow the planted construction f(x) = q(x)^2 + r * prod_i (b_i x - a_i):
+at x = a_i / b_i the product vanishes, so F(a_i, b_i) = (b_i^m q(a_i/b_i))^2
+is a perfect square and (a_i : y : b_i) is a rational point.  Random draws use
+SplitMix64 so every workload is reproducible from its seed alone.
+"""
+
+import math
+from fractions import Fraction
+
+
+class SplitMix64:
+    def __init__(self, seed):
+        self.state = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self):
+        self.state = (self.state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self, lo, hi):
+        """Integer uniform in [lo, hi] (rejection sampling on 64-bit words)."""
+        span = hi - lo + 1
+        if span <= 0:
+            raise ValueError("empty range")
+        bits = span.bit_length()
+        while True:
+            value = 0
+            for _ in range((bits + 63) // 64):
+                value = (value << 64) | self.next()
+            value &= (1 << bits) - 1
+            if value < span:
+                return lo + value
+
+
+def poly_mul(p, q):
+    out = [0] * (len(p) + len(q) - 1)
+    for i, a in enumerate(p):
+        if a:
+            for j, b in enumerate(q):
+                out[i + j] += a * b
+    return out
+
+
+def poly_add(p, q):
+    n = max(len(p), len(q))
+    return [(p[i] if i < len(p) else 0) + (q[i] if i < len(q) else 0) for i in range(n)]
+
+
+def _trim(p):
+    while p and p[-1] == 0:
+        p.pop()
+    return p
+
+
+def is_squarefree(coeffs):
+    """gcd(f, f') is constant, by Euclid over Q (the test of
+    src/exact_polynomial.cpp:44-71 in the reference)."""
+    f = _trim([Fraction(c) for c in coeffs])
+    g = _trim([f[i] * i for i in range(1, len(f))])
+    while g:
+        r = list(f)
+        while len(r) >= len(g):
+            shift = len(r) - len(g)
+            factor = r[-1] / g[-1]
+            for i in range(len(g)):
+                r[i + shift] -= factor * g[i]
+            _trim(r)
+            if not r:
+                break
+        f, g = g, r
+    return len(f) == 1
+
+
+def planted_curve(seed, q_degree, q_bound, point_bound, n_points, r_bound=100,
+                  degree=None):
+    """Draws f = q^2 + r * prod (b_i x - a_i) with coprime planted points.
+
+    Returns (coefficients c0..cn as ints, points [(a_i, b_i)]).  Redraws
+    until f is squarefree and has the expected degree."""
+    rng = SplitMix64(seed)
+    target = degree if degree is not None else 2 * q_degree
+    while True:
+        q = [rng.uniform(-q_bound, q_bound) for _ in range(q_degree + 1)]
+        if q[-1] == 0:
+            continue
+        r = 0
+        while r == 0:
+            r = rng.uniform(-r_bound, r_bound)
+        points = []
+        while len(points) < n_points:
+            a = rng.uniform(-point_bound, point_bound)
+            b = rng.uniform(1, point_bound)
+            if math.gcd(a, b) == 1 and (a, b) not in points:
+                points.append((a, b))
+        prod = [1]
+        for a, b in points:
+            prod = poly_mul(prod, [-a, b])
+        f = poly_add(poly_mul(q, q), [r * c for c in prod])
+        _trim(f)
+        if len(f) - 1 != target:
+            continue
+        if is_squarefree(f):
+            return f, points
+
+
+def config1_curve(seed=1):
+    """Plain method, small: sextic, q coefficients in [-2^15, 2^15], six
+    planted points with |a_i| <= 1000, 1 <= b_i <= 1000; box A = B = 1000."""
+    return planted_curve(seed, 3, 2 ** 15, 1000, 6)
+
+
+def config2_curve(seed=2):
+    """Plain method at full size: q coefficients in [-2^30, 2^30], planted
+    |a_i|, b_i <= 5e6 (coefficients of roughly 150 bits); box A = B = 5e6."""
+    return planted_curve(seed, 3, 2 ** 30, 5_000_000, 6)
+
+
+CONFIG1 = {"name": "planted-sextic-A1000-B1000", "seed": 1, "bound": 1000,
+           "first": 1, "last": 1000}
+CONFIG2 = {"name": "planted-sextic-A5e6-B5e6", "seed": 2, "bound": 5_000_000,
+           "first": 1, "last": 5_000_000, "batch": 65536}
+
+RECORD_CURVE = [247747600, -985905640, 567207969, 2396040466, 52485681,
+                -470135160, 82342800]
+
+
+def sites(bound, first, last):
+    return (2 * bound + 1) * (last - first + 1)

@@ -1,0 +1,62 @@
+"""The C-ABI library loads and exports every symbol include/*.h declares
+(no compute calls; runs without a GPU)."""
+
+import ctypes
+import glob
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for header in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(header).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for match in re.finditer(r"\b(rps_[a-z_0-9]+)\s*\(", text):
+            names.add(match.group(1))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for required in ("rps_open", "rps_close", "rps_sieve", "rps_last_error"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol(pkg):
+    path = pkg.LIB_PATH
+    assert os.path.exists(path), "run __graft_entry__.build() first"
+    lib = ctypes.CDLL(path)
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (rps_\w+)", out))
+    assert set(declared_functions()) <= exported
+
+
+def test_variants_listed(pkg):
+    names = pkg.variants()
+    assert names and names[0] == "u2_f14_w8"
+    for n in names:
+        assert re.fullmatch(r"u\d_f\d+_w\d+", n)
+
+
+def test_library_targets_gfx950(pkg):
+    out = subprocess.run(["strings", pkg.LIB_PATH], capture_output=True, text=True).stdout
+    assert "gfx950" in out
+
+
+def test_missing_library_fails_loudly(pkg, tmp_path):
+    saved = pkg._LIB
+    pkg._LIB = None
+    try:
+        with pytest.raises(RuntimeError, match="not built"):
+            pkg.load_library(str(tmp_path / "nope.so"))
+    finally:
+        pkg._LIB = saved

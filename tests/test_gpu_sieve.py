@@ -1,0 +1,240 @@
+"""GPU parity of the HIP sieve against the CPU oracle, through the C ABI.
+
+Bar: the survivor multiset is bit-identical (sorted arrays equal) to the
+oracle's on every case the oracle finishes in seconds; at full sizes the
+survivor counts equal the counts the reference holds, and size-independent
+properties hold (monotonicity in A, soundness of every survivor checked by
+Euler's criterion, completeness on windows checked exhaustively).
+"""
+
+import json
+import math
+import os
+import random
+
+import numpy as np
+import pytest
+
+import curves
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+ROWS = {r["name"]: r for r in json.load(open(os.path.join(GOLDEN, "reference_survivor_counts.json")))}
+
+
+def announce(**case):
+    print("case:", " ".join(f"{k}={v}" for k, v in case.items()), flush=True)
+
+
+def assert_same(gpu_result, oracle_result):
+    count, digest, num, den = oracle_result
+    assert gpu_result.survivor_count == count
+    assert np.array_equal(gpu_result.numerators, num)
+    assert np.array_equal(gpu_result.denominators, den)
+    assert oracle.site_digest(gpu_result.numerators, gpu_result.denominators) == digest
+
+
+def test_dense_streaming_exactly_once(fresh_gpu):
+    # tests/test_sieve_streaming.cpp: 20,002 survivors through several
+    # flushes of a buffer smaller than the survivor set.
+    announce(case="dense", capacity=16384)
+    fresh_gpu.set_option("survivor_capacity", 16384)
+    res = fresh_gpu.sieve(["1", str(curves.dense_product())], 5000, 1, 2)
+    assert res.survivor_count == 20002
+    expected_num = np.concatenate([np.arange(-5000, 5001)] * 2)
+    expected_den = np.repeat(np.array([1, 2], dtype=np.int32), 10001)
+    assert np.array_equal(res.numerators, expected_num)
+    assert np.array_equal(res.denominators, expected_den)
+    assert res.stats["launches"] >= 2
+
+
+def test_dense_many_flushes(fresh_gpu, oracle_lib):
+    announce(case="dense-many-flushes", capacity=16384)
+    fresh_gpu.set_option("survivor_capacity", 16384)
+    coeffs = ["1", str(curves.dense_product())]
+    res = fresh_gpu.sieve(coeffs, 40000, 3, 7)
+    assert res.survivor_count == 5 * 80001
+    assert res.stats["launches"] >= 5 * 80001 // 16384
+    assert_same(res, oracle_lib.sieve(coeffs, 40000, 3, 7, method="words"))
+
+
+FAST_GOLDEN = ["dense-1-P", "rank31-gpu-fixed-validation-p0", "rank31-gpu-fixed-validation-grid",
+               "rank31-gpu-fixed-benchmark-1m-64", "rank31-gpu-fixed-p0-10m-64",
+               "rank31-gpu-fixed-pd-100m-256", "rank31-gpu-fixed-s-1b-1024",
+               "rank31-gpu-fixed-p0-2b-8192", "record-screen-10000000-1000000"]
+
+
+@pytest.mark.parametrize("name", FAST_GOLDEN)
+def test_reference_survivor_counts(fresh_gpu, name):
+    row = ROWS[name]
+    announce(case=name, A=row["bound"], b=f"{row['first']}..{row['last']}")
+    rc, stats = fresh_gpu.run(row["coefficients"].split(), row["bound"], row["first"],
+                              row["last"], row["square"], lambda n, d: False)
+    assert stats["survivor_count"] == row["survivors"]
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_random_curves_match_oracle(fresh_gpu, oracle_lib, seed):
+    rng = random.Random(seed)
+    degree = rng.choice([1, 2, 3, 4, 5, 6, 7, 8, 10, 12])
+    coeffs, live = curves.partially_dense(rng, rng.randint(3, 8), degree,
+                                          coeff_bound=rng.choice([10, 10 ** 6, 10 ** 30]))
+    bound = rng.choice([1, 2, 31, 63, 64, 65, 127, 128, 1000, rng.randint(1, 50000)])
+    first = rng.choice([1, rng.randint(1, 10 ** 5), 2 ** 31 - 300])
+    last = min(2 ** 31 - 1, first + rng.randint(0, 200))
+    announce(seed=seed, degree=degree, A=bound, b=f"{first}..{last}")
+    res = fresh_gpu.sieve(coeffs, bound, first, last)
+    assert_same(res, oracle_lib.sieve(coeffs, bound, first, last, method="words"))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_square_denominators_match_oracle(fresh_gpu, oracle_lib, seed):
+    rng = random.Random(100 + seed)
+    coeffs, _ = curves.partially_dense(rng, rng.randint(3, 7), rng.choice([3, 4]))
+    first = rng.choice([1, 46340 - 150, rng.randint(1, 46000)])
+    last = min(46340, first + 150)
+    bound = rng.randint(1, 20000)
+    announce(seed=seed, square=True, A=bound, k=f"{first}..{last}")
+    res = fresh_gpu.sieve(coeffs, bound, first, last, square=True)
+    assert_same(res, oracle_lib.sieve(coeffs, bound, first, last, square=True, method="words"))
+
+
+def test_square_mode_golden_point(fresh_gpu):
+    # The point (1 : 14 : 4) of y^2 = 1 - x + x^3 at k = 2 must survive.
+    announce(case="square-golden")
+    res = fresh_gpu.sieve(["1", "-1", "0", "1"], 10, 1, 3, square=True)
+    pairs = set(zip(res.numerators.tolist(), res.denominators.tolist()))
+    assert (1, 4) in pairs and all(d in (1, 4, 9) for _, d in pairs)
+
+
+def test_divisible_denominators(fresh_gpu, oracle_lib):
+    # Denominators divisible by one to three candidate primes use a
+    # different prime set.
+    coeffs, _ = curves.partially_dense(random.Random(5), 6, 6)
+    for b in (509, 293 * 307, 293 * 307 * 311, 499 * 503 * 509 - 1, 2 * 3 * 467 * 499):
+        announce(case="divisible", b=b)
+        res = fresh_gpu.sieve(coeffs, 30000, b, b + 2)
+        assert_same(res, oracle_lib.sieve(coeffs, 30000, b, b + 2, method="words"))
+
+
+@pytest.mark.parametrize("variant", range(8))
+def test_variants_agree(fresh_gpu, oracle_lib, pkg, variant):
+    if variant >= len(pkg.variants()):
+        pytest.skip("variant not compiled")
+    coeffs, _ = curves.partially_dense(random.Random(77), 7, 6)
+    announce(case="variant", variant=pkg.variants()[variant])
+    fresh_gpu.set_option("variant", variant)
+    res = fresh_gpu.sieve(coeffs, 70001, 11, 140)
+    assert_same(res, oracle_lib.sieve(coeffs, 70001, 11, 140, method="words"))
+
+
+def test_line_split_into_chunks(fresh_gpu, oracle_lib):
+    # Few denominators and a long line: each line is cut into many tiles.
+    coeffs, _ = curves.partially_dense(random.Random(8), 9, 4)
+    announce(case="chunks", A=3_000_001, b="1..3")
+    res = fresh_gpu.sieve(coeffs, 3_000_001, 1, 3)
+    assert res.stats["launches"] == 1
+    assert_same(res, oracle_lib.sieve(coeffs, 3_000_001, 1, 3, method="words"))
+
+
+def test_small_batches(fresh_gpu, oracle_lib):
+    coeffs, _ = curves.partially_dense(random.Random(9), 6, 5)
+    fresh_gpu.set_option("batch_size", 7)
+    announce(case="batch7")
+    res = fresh_gpu.sieve(coeffs, 5000, 1, 50)
+    assert res.stats["batches"] == 8
+    assert_same(res, oracle_lib.sieve(coeffs, 5000, 1, 50, method="words"))
+
+
+def test_huge_bound_properties(fresh_gpu):
+    # A = 2^40 + 12345 (3.4e10 words per line, so 64-bit word indices and
+    # line chunks are exercised).  Checks: a sample of survivors and every
+    # survivor near the line ends pass Euler's criterion (soundness); the
+    # windows |a| <= 2000 and |a| >= A - 2000 are exhaustively complete; and
+    # a second run with A2 = A - 777 (a different word alignment) gives the
+    # same survivors on |a| <= A2.
+    rng = random.Random(12)
+    coeffs, _ = curves.partially_dense(rng, 22, 3)
+    ints = [int(c) for c in coeffs]
+    bound = 2 ** 40 + 12345
+    announce(case="huge-bound", A=bound)
+    res = fresh_gpu.sieve(coeffs, bound, 5, 6)
+    num, den = res.numerators, res.denominators
+    assert 0 < res.survivor_count < 20_000_000
+    assert np.all(np.abs(num) <= bound)
+
+    def passes(a, b):
+        v = oracle.homogenised_value(ints, a, b)
+        return all(pow(v % p, (p - 1) // 2, p) in (0, 1) for p in oracle.selected_primes(b))
+
+    picks = np.random.default_rng(0).choice(len(num), size=min(2000, len(num)), replace=False)
+    ends = np.nonzero(np.abs(num) > bound - 10 ** 7)[0]
+    for i in list(picks) + list(ends):
+        assert passes(int(num[i]), int(den[i]))
+    got = set(zip(num[np.abs(num) <= 2000].tolist(), den[np.abs(num) <= 2000].tolist()))
+    got |= set(zip(num[np.abs(num) >= bound - 2000].tolist(),
+                   den[np.abs(num) >= bound - 2000].tolist()))
+    for b in (5, 6):
+        for a in list(range(-2000, 2001)) + list(range(bound - 2000, bound + 1)) \
+                + list(range(-bound, -bound + 2001)):
+            assert passes(a, b) == ((a, b) in got)
+    bound2 = bound - 777
+    res2 = fresh_gpu.sieve(coeffs, bound2, 5, 6)
+    keep = np.abs(num) <= bound2
+    assert np.array_equal(num[keep], res2.numerators)
+    assert np.array_equal(den[keep], res2.denominators)
+
+
+def test_monotone_in_bound(fresh_gpu):
+    # The predicate does not depend on A, so S(A') = S(A) restricted to
+    # |a| <= A'.
+    coeffs, _ = curves.partially_dense(random.Random(13), 10, 6)
+    announce(case="monotone")
+    big = fresh_gpu.sieve(coeffs, 2_000_000, 100, 130)
+    small = fresh_gpu.sieve(coeffs, 777_777, 100, 130)
+    keep = np.abs(big.numerators) <= 777_777
+    assert np.array_equal(big.numerators[keep], small.numerators)
+    assert np.array_equal(big.denominators[keep], small.denominators)
+
+
+def test_planted_points_survive(fresh_gpu, workloads):
+    coeffs, points = workloads.config2_curve()
+    announce(case="planted", A=5_000_000)
+    res = fresh_gpu.sieve(coeffs, 5_000_000, 1, 2000)
+    got = set(zip(res.numerators.tolist(), res.denominators.tolist()))
+    for a, b in points:
+        if b <= 2000:
+            assert (a, b) in got
+
+
+def test_cancel_stops(fresh_gpu):
+    announce(case="cancel")
+    fresh_gpu.set_option("survivor_capacity", 16384)
+    fresh_gpu.set_option("batch_size", 1)
+    calls = []
+    rc, stats = fresh_gpu.run(["1", str(curves.dense_product())], 100000, 1, 50, False,
+                              lambda n, d: calls.append(len(n)) or True)
+    assert rc == 4 and len(calls) == 1
+
+
+def test_invalid_requests(fresh_gpu, pkg):
+    for args in ((["1", "1"], 10, 0, 5), (["1", "1"], 10, 6, 5), (["1", "1"], 0, 1, 5),
+                 (["1"], 10, 1, 5), (["1", "x"], 10, 1, 5)):
+        with pytest.raises(pkg.SieveError) as err:
+            fresh_gpu.run(*args, sink=lambda n, d: False)
+        assert err.value.code == 1
+    with pytest.raises(pkg.SieveError) as err:
+        fresh_gpu.run(["1", "1"], 10, 1, 46341, True, lambda n, d: False)
+    assert str(err.value) == "invalid denominator range"
+
+
+def test_curve_cache_switch(fresh_gpu, oracle_lib):
+    # Alternating curves must rebuild the per-curve table each time.
+    rng = random.Random(21)
+    c1, _ = curves.partially_dense(rng, 6, 4)
+    c2, _ = curves.partially_dense(rng, 6, 6)
+    for coeffs in (c1, c2, c1):
+        res = fresh_gpu.sieve(coeffs, 4000, 1, 40)
+        assert_same(res, oracle_lib.sieve(coeffs, 4000, 1, 40, method="words"))
