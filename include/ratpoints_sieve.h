@@ -109,6 +109,21 @@ const char *rps_last_error(const rps_ctx *ctx);
 int rps_sieve(rps_ctx *ctx, const rps_request *request, rps_sink sink,
               void *user, rps_stats *stats);
 
+/* Batch-completion notice: every survivor with a denominator parameter in
+ * [first_t, last_t] has been delivered to the sink, and batches complete in
+ * increasing parameter order.  Return 0 to continue; nonzero cancels the
+ * request like a nonzero sink return.  No counterpart in the reference, whose
+ * caller sees one batch per run_modular_sieve call
+ * (src/point_search.cpp:195-237); it lets a caller sieve a whole range in one
+ * call -- so the GPU works on batch i+1 while the caller verifies batch i --
+ * and still emit points in (b, a) order and stop early for -1. */
+typedef int (*rps_batch_done)(void *user, int32_t first_t, int32_t last_t,
+                              uint64_t survivors);
+
+/* rps_sieve with a batch-completion callback (`done` may be NULL). */
+int rps_sieve_batched(rps_ctx *ctx, const rps_request *request, rps_sink sink,
+                      rps_batch_done done, void *user, rps_stats *stats);
+
 /* Tuning and test knobs; returns RPS_EINVAL for an unknown name or a value
  * out of range.  No counterpart in the reference, whose knobs were
  * compile-time macros (src/sieve_plan.hpp:10-22, src/sieve.cu:15-17):

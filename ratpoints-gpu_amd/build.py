@@ -69,7 +69,9 @@ def build_library():
     return out
 
 
-def build_cli():
+def build_cli(backend=None, out=None):
+    """The CLI over the HIP library, or -- for tests only -- over another
+    implementation of the C ABI given as a list of object files."""
     host = os.path.join(HERE, "host")
     sources = [os.path.join(host, s) for s in HOST_SOURCES]
     if not all(os.path.exists(s) for s in sources):
@@ -77,18 +79,35 @@ def build_cli():
     prefix = gmp_prefix()
     if prefix is None:
         raise RuntimeError("GMP (gmpxx.h) not found under /usr, /usr/local or /opt/conda")
-    os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
-    out = os.path.join(HERE, "bin", "ratpoints_gpu")
+    out = out or os.path.join(HERE, "bin", "ratpoints_gpu")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
     libdir = os.path.join(HERE, "lib")
     cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-Wextra", "-pthread",
            "-I" + host, "-I" + os.path.join(ROOT, "include")]
+    rpath = "-Wl,-rpath,$ORIGIN/../lib"
     if prefix != "/usr":
+        # GMP is linked by full path and found at run time through a
+        # directory of symlinks to it alone, so the prefix's other libraries
+        # (an older libstdc++ under /opt/conda) never shadow the system ones.
+        gmp_dir = os.path.join(libdir, "gmp")
+        os.makedirs(gmp_dir, exist_ok=True)
+        for soname in ("libgmp.so.10", "libgmpxx.so.4"):
+            link = os.path.join(gmp_dir, soname)
+            if os.path.lexists(link):
+                os.remove(link)
+            os.symlink(os.path.join(prefix, "lib", soname), link)
         cmd += ["-I" + os.path.join(prefix, "include")]
-    cmd += [*sources, "-o", out + ".tmp", "-L" + libdir, "-lratpoints_sieve",
-            "-Wl,-rpath,$ORIGIN/../lib"]
-    if prefix != "/usr":
-        cmd += ["-L" + os.path.join(prefix, "lib"), "-Wl,-rpath," + os.path.join(prefix, "lib")]
-    cmd += ["-lgmpxx", "-lgmp"]
+        rpath += ",-rpath,$ORIGIN/../lib/gmp"
+        gmp_libs = [os.path.join(prefix, "lib", "libgmpxx.so"),
+                    os.path.join(prefix, "lib", "libgmp.so")]
+    else:
+        gmp_libs = ["-lgmpxx", "-lgmp"]
+    if backend is None:
+        abi = ["-L" + libdir, "-lratpoints_sieve"]
+    else:
+        abi = list(backend)
+        rpath = rpath.replace("$ORIGIN/../lib", libdir)
+    cmd += [*sources, "-o", out + ".tmp", *abi, rpath, *gmp_libs]
     run(cmd)
     os.replace(out + ".tmp", out)
     return out
@@ -101,6 +120,17 @@ def build_oracle():
     run(["make", "-B", "-C", oracle])
     if os.path.isdir("/root/reference/src"):
         run(["make", "-B", "-C", oracle, "ref"])
+    # The host CLI over the oracle's CPU test double of the C ABI, for the
+    # host-logic tests (never shipped; see oracle/abi_oracle_backend.c).
+    build_dir = os.path.join(oracle, "_build")
+    os.makedirs(build_dir, exist_ok=True)
+    objs = []
+    for src in ("abi_oracle_backend.c", "sieve_oracle.c"):
+        obj = os.path.join(build_dir, src[:-2] + ".o")
+        run(["gcc", "-O2", "-std=c11", "-fPIC", "-pthread", "-I" + os.path.join(ROOT, "include"),
+             "-c", os.path.join(oracle, src), "-o", obj])
+        objs.append(obj)
+    build_cli(backend=objs, out=os.path.join(build_dir, "ratpoints_gpu_oracle"))
 
 
 def build_all():

@@ -155,14 +155,14 @@ struct rps_ctx {
 namespace {
 
 void free_slot(Slot &s) {
-    if (s.d_state) hipFree(s.d_state);
-    if (s.h_state) hipHostFree(s.h_state);
-    if (s.d_num) hipFree(s.d_num);
-    if (s.d_den) hipFree(s.d_den);
-    if (s.d_cursors) hipFree(s.d_cursors);
-    if (s.k_start) hipEventDestroy(s.k_start);
-    if (s.k_end) hipEventDestroy(s.k_end);
-    if (s.ready) hipEventDestroy(s.ready);
+    if (s.d_state) (void)hipFree(s.d_state);
+    if (s.h_state) (void)hipHostFree(s.h_state);
+    if (s.d_num) (void)hipFree(s.d_num);
+    if (s.d_den) (void)hipFree(s.d_den);
+    if (s.d_cursors) (void)hipFree(s.d_cursors);
+    if (s.k_start) (void)hipEventDestroy(s.k_start);
+    if (s.k_end) (void)hipEventDestroy(s.k_end);
+    if (s.ready) (void)hipEventDestroy(s.ready);
     s = Slot{};
 }
 
@@ -330,22 +330,22 @@ void rps_close(rps_ctx *ctx) {
     if (!ctx) {
         return;
     }
-    hipSetDevice(ctx->device);
-    if (ctx->compute) hipStreamSynchronize(ctx->compute);
-    if (ctx->copy) hipStreamSynchronize(ctx->copy);
+    (void)hipSetDevice(ctx->device);
+    if (ctx->compute) (void)hipStreamSynchronize(ctx->compute);
+    if (ctx->copy) (void)hipStreamSynchronize(ctx->copy);
     for (Slot &s : ctx->slots) {
         free_slot(s);
     }
-    if (ctx->d_primes) hipFree(ctx->d_primes);
-    if (ctx->d_coef_res) hipFree(ctx->d_coef_res);
-    if (ctx->d_chi) hipFree(ctx->d_chi);
-    if (ctx->d_resid) hipFree(ctx->d_resid);
-    if (ctx->h_num) hipHostFree(ctx->h_num);
-    if (ctx->h_den) hipHostFree(ctx->h_den);
-    if (ctx->basis_start) hipEventDestroy(ctx->basis_start);
-    if (ctx->basis_end) hipEventDestroy(ctx->basis_end);
-    if (ctx->compute) hipStreamDestroy(ctx->compute);
-    if (ctx->copy) hipStreamDestroy(ctx->copy);
+    if (ctx->d_primes) (void)hipFree(ctx->d_primes);
+    if (ctx->d_coef_res) (void)hipFree(ctx->d_coef_res);
+    if (ctx->d_chi) (void)hipFree(ctx->d_chi);
+    if (ctx->d_resid) (void)hipFree(ctx->d_resid);
+    if (ctx->h_num) (void)hipHostFree(ctx->h_num);
+    if (ctx->h_den) (void)hipHostFree(ctx->h_den);
+    if (ctx->basis_start) (void)hipEventDestroy(ctx->basis_start);
+    if (ctx->basis_end) (void)hipEventDestroy(ctx->basis_end);
+    if (ctx->compute) (void)hipStreamDestroy(ctx->compute);
+    if (ctx->copy) (void)hipStreamDestroy(ctx->copy);
     delete ctx;
 }
 
@@ -449,6 +449,11 @@ int rps_last_kernel_timing(const rps_ctx *ctx, double *kernel_ms,
 
 int rps_sieve(rps_ctx *ctx, const rps_request *req, rps_sink sink, void *user,
               rps_stats *stats) {
+    return rps_sieve_batched(ctx, req, sink, nullptr, user, stats);
+}
+
+int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
+                      rps_batch_done done, void *user, rps_stats *stats) {
     if (!ctx) {
         return RPS_EINVAL;
     }
@@ -613,6 +618,7 @@ int rps_sieve(rps_ctx *ctx, const rps_request *req, rps_sink sink, void *user,
         }
         int cur = 0;
         while (busy[cur] && !cancelled) {
+            const uint64_t before = st.survivor_count;
             bool more = drain(cur);
             while (more && !cancelled) {
                 launch(cur, inflight[cur], true);
@@ -620,6 +626,14 @@ int rps_sieve(rps_ctx *ctx, const rps_request *req, rps_sink sink, void *user,
             }
             if (cancelled) {
                 break;
+            }
+            if (done) {
+                const Batch &bt = inflight[cur];
+                if (done(user, bt.first_t, bt.first_t + bt.n_t - 1,
+                         st.survivor_count - before) != 0) {
+                    cancelled = true;
+                    break;
+                }
             }
             busy[cur] = next_batch(inflight[cur]);
             if (busy[cur]) {
@@ -638,14 +652,14 @@ int rps_sieve(rps_ctx *ctx, const rps_request *req, rps_sink sink, void *user,
         return cancelled ? RPS_ECANCELLED : RPS_OK;
     } catch (const HipError &e) {
         ctx->error = op_name(e.op) + ": " + hipGetErrorString(e.code);
-        hipStreamSynchronize(ctx->compute);
-        hipStreamSynchronize(ctx->copy);
+        (void)hipStreamSynchronize(ctx->compute);
+        (void)hipStreamSynchronize(ctx->copy);
         ctx->curve_ready = false;
         return RPS_EDEVICE;
     } catch (const ApiError &e) {
         ctx->error = e.message;
-        hipStreamSynchronize(ctx->compute);
-        hipStreamSynchronize(ctx->copy);
+        (void)hipStreamSynchronize(ctx->compute);
+        (void)hipStreamSynchronize(ctx->copy);
         return e.code;
     } catch (const std::bad_alloc &) {
         ctx->error = "out of host memory";

@@ -1,0 +1,110 @@
+// Device discovery over the C ABI (rps_device_count / rps_device_info).
+// Selection syntax and messages follow src/gpu_devices.cpp:21-97 of the
+// reference; a search uses exactly one device.
+#include "gpu_devices.hpp"
+
+#include <algorithm>
+#include <limits>
+#include <stdexcept>
+
+#include "ratpoints_sieve.h"
+
+namespace ratpoints_gpu {
+namespace {
+
+thread_local int bound_device = 0;
+
+}  // namespace
+
+std::vector<int> parse_device_selection(const std::string &text) {
+    if (text == "all") {
+        return {};
+    }
+    std::vector<int> ids;
+    size_t at = 0;
+    while (at < text.size()) {
+        size_t comma = text.find(',', at);
+        if (comma == std::string::npos) {
+            comma = text.size();
+        }
+        if (comma == at) {
+            throw std::invalid_argument("empty GPU ID in --devices");
+        }
+        int id = 0;
+        for (size_t i = at; i < comma; ++i) {
+            const int digit = text[i] - '0';
+            if (digit < 0 || digit > 9
+                || id > (std::numeric_limits<int>::max() - digit) / 10) {
+                throw std::invalid_argument(
+                    "--devices requires all or comma-separated nonnegative GPU IDs");
+            }
+            id = 10 * id + digit;
+        }
+        if (std::find(ids.begin(), ids.end(), id) != ids.end()) {
+            throw std::invalid_argument("duplicate GPU ID in --devices");
+        }
+        ids.push_back(id);
+        at = comma + 1;
+    }
+    if (ids.empty() || text.back() == ',') {
+        throw std::invalid_argument("--devices requires all or a nonempty GPU list");
+    }
+    return ids;
+}
+
+std::vector<GpuDevice> available_gpu_devices() {
+    int count = 0;
+    if (rps_device_count(&count) != RPS_OK) {
+        throw std::runtime_error(rps_last_error(nullptr));
+    }
+    if (count == 0) {
+        throw std::runtime_error(
+            "no visible HIP devices; check the driver and HIP_VISIBLE_DEVICES");
+    }
+    std::vector<GpuDevice> devices;
+    for (int id = 0; id < count; ++id) {
+        char name[256] = {0};
+        char pci[64] = {0};
+        int units = 0, major = 0, minor = 0;
+        uint64_t memory = 0;
+        if (rps_device_info(id, name, sizeof(name), pci, sizeof(pci), &units,
+                            &memory, &major, &minor) != RPS_OK) {
+            throw std::runtime_error(rps_last_error(nullptr));
+        }
+        devices.push_back({id, name, major, minor, units,
+                           static_cast<unsigned long long>(memory), pci});
+    }
+    return devices;
+}
+
+std::vector<GpuDevice> select_gpu_devices(const std::vector<int> &ids) {
+    std::vector<GpuDevice> visible = available_gpu_devices();
+    if (ids.empty()) {
+        return {visible.front()};
+    }
+    if (ids.size() > 1) {
+        throw std::invalid_argument(
+            "this backend searches on one GPU; select it with --devices N");
+    }
+    const int id = ids.front();
+    if (id < 0 || static_cast<size_t>(id) >= visible.size()) {
+        throw std::invalid_argument("GPU ID " + std::to_string(id)
+            + " is not visible; run --list-devices (IDs are HIP-visible ordinals)");
+    }
+    return {visible[static_cast<size_t>(id)]};
+}
+
+void activate_gpu(int id) {
+    int count = 0;
+    if (rps_device_count(&count) != RPS_OK) {
+        throw std::runtime_error(rps_last_error(nullptr));
+    }
+    if (id < 0 || id >= count) {
+        throw std::runtime_error("hipSetDevice: invalid device ordinal");
+    }
+    bound_device = id;
+}
+
+int active_gpu() { return bound_device; }
+
+}  // namespace ratpoints_gpu
