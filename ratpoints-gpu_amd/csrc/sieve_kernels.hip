@@ -44,6 +44,12 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
+    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(v), lane);
+    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(v >> 32), lane);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 }  // namespace
 
 // One workgroup of 512 threads per candidate prime.  Thread x evaluates
@@ -267,6 +273,21 @@ __global__ __launch_bounds__(WAVES * 64) void sieve_kernel(
         }
     }
 
+    // Late-prime constants: lane j of each 32-lane half owns prime FAST + j.
+    constexpr int kLate = kSelected - FAST;
+    static_assert(kLate <= 32, "late primes must fit a half wave");
+    const int lj = lane & 31;
+    const bool late_lane = lj < kLate;
+    uint32_t lp = 1, lmagic = 0, lbase = 0, lc = 0;
+    const uint64_t *lcx = cext;
+    if (late_lane) {
+        lp = s_p[FAST + lj];
+        lmagic = s_magic[FAST + lj];
+        lbase = s_base[FAST + lj];
+        lc = s_c[FAST + lj];
+        lcx = cext + (FAST + lj) * 10;
+    }
+
     const uint64_t last_word = P.words - 1;
     const uint64_t neg_a = static_cast<uint64_t>(-P.bound);
     bool stopped = false;
@@ -308,28 +329,41 @@ __global__ __launch_bounds__(WAVES * 64) void sieve_kernel(
         if (!__ballot(any != 0)) {
             continue;
         }
-        // Remaining primes, only for words that still have bits set.
-        const uint32_t rel = static_cast<uint32_t>(W - ts) + lane;
-        for (int s = FAST; s < kSelected; ++s) {
-            if (!__ballot(any != 0)) {
-                break;
-            }
-            if (any) {
-                const uint32_t p = s_p[s];
-                const uint32_t magic = s_magic[s];
-                const uint32_t base = s_base[s];
-                const uint32_t c = s_c[s];
-                const uint64_t *cx = cext + s * 10;
-                any = 0;
+        // Remaining primes, only for words that still have bits set.  Up to
+        // two such words are taken per pass as wave-uniform values; in each
+        // 32-lane half, lane j evaluates late prime FAST + j for its word and
+        // the half ANDs the row words together.
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    if (m[u]) {
-                        uint32_t wm = mod_small(rel + 64 * u, p, magic) + base;
-                        wm = wm >= p ? wm - p : wm;
-                        const uint32_t o = mod_small(c + 64 * wm, p, magic);
-                        m[u] &= window64(cx, o);
-                    }
-                    any |= m[u];
+        for (int u = 0; u < U; ++u) {
+            uint64_t pend = __ballot(m[u] != 0);
+            while (pend) {
+                const int l0 = __builtin_ctzll(pend);
+                pend &= pend - 1;
+                int l1 = -1;
+                if (pend) {
+                    l1 = __builtin_ctzll(pend);
+                    pend &= pend - 1;
+                }
+                const int src = lane < 32 ? l0 : l1;
+                uint64_t e = ~0ULL;
+                if (late_lane && src >= 0) {
+                    const uint32_t rel =
+                        static_cast<uint32_t>(W - ts) + src + 64u * u;
+                    uint32_t wm = mod_small(rel, lp, lmagic) + lbase;
+                    wm = wm >= lp ? wm - lp : wm;
+                    e = window64(lcx, mod_small(lc + 64u * wm, lp, lmagic));
+                }
+#pragma unroll
+                for (int d = 1; d < 32; d <<= 1) {
+                    e &= __shfl_xor(e, d, 64);
+                }
+                const uint64_t r0 = readlane64(e, 0);
+                const uint64_t r1 = readlane64(e, 32);
+                if (lane == l0) {
+                    m[u] &= r0;
+                }
+                if (lane == l1) {
+                    m[u] &= r1;
                 }
             }
         }

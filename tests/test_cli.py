@@ -43,7 +43,7 @@ def run(binary, args, env=None, ok=True):
     full_env.pop("RATPOINTS_GPU_BENCHMARK", None)
     full_env.update(env or {})
     print("cli:", os.path.basename(binary), args[:1], args[1:8], flush=True)
-    proc = subprocess.run([binary, *args], capture_output=True, timeout=120, env=full_env)
+    proc = subprocess.run([binary, *args], capture_output=True, timeout=300, env=full_env)
     assert (proc.returncode == 0) == ok, (proc.returncode, proc.stderr)
     return proc
 
@@ -244,3 +244,24 @@ def test_config1_planted_points(workloads):
     # oracle's exact stage on the first 60 denominators instead.
     sub = run(PRODUCT, [text, "1000", "-du", "60", "-q"])
     assert sub.stdout == oracle.render_output(coeffs, 1000, 1, 60)
+
+
+GOLDEN_ROWS = {r["name"]: r for r in __import__("json").load(
+    open(os.path.join(ROOT, "tests", "golden", "reference_survivor_counts.json")))}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["record-screen-10000000-1000000",
+                                  "record-num-primes-10000000-10000000"])
+def test_record_curve_stdout_identical_to_reference(name):
+    # docs/RTX4090-SCREEN.json, docs/RTX4090-NUM-PRIMES.json: the reference
+    # binary's stdout SHA-256 and survivor counts for the record curve at
+    # height 1e7 (tests/benchmark_multi_gpu.py:73-78 is the command).
+    import hashlib
+    row = GOLDEN_ROWS[name]
+    proc = run(PRODUCT, [row["coefficients"], str(row["bound"]), "-dl", str(row["first"]),
+                         "-du", str(row["last"]), "--batch-size", "65536", "-v"],
+               env={"RATPOINTS_GPU_BENCHMARK": "1"})
+    assert hashlib.sha256(proc.stdout).hexdigest() == row["output_sha256"]
+    m = re.search(rb"survivors=(\d+) exact_survivors=(\d+)", proc.stderr)
+    assert (int(m[1]), int(m[2])) == (row["survivors"], row["exact"])
