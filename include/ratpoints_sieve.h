@@ -132,6 +132,8 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *request, rps_sink sink,
  *   "variant"            kernel variant index, 0..rps_variant_count()-1
  *   "min_tiles"          split denominators into word chunks until at least
  *                        this many tiles per launch exist (default 2048)
+ *   "direct_words"       lines of at most this many 64-bit words use the
+ *                        short-line kernel, 0..768 (default 768; 0 = never)
  */
 int rps_set_option(rps_ctx *ctx, const char *name, int64_t value);
 

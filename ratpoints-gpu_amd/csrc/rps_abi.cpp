@@ -121,6 +121,8 @@ struct Batch {
     int tiles_per_t = 1;
     unsigned long long tile_words = 0;
     size_t tiles = 0;
+    bool direct = false;  // short lines: tiles of tile_t whole lines
+    int tile_t = 1;
 };
 
 }  // namespace
@@ -146,6 +148,7 @@ struct rps_ctx {
     int batch_size = 1 << 16;
     int variant = 0;
     long long min_tiles = 2048;
+    unsigned long long direct_words = rps::kDirectMaxWords;
     hipEvent_t basis_start = nullptr, basis_end = nullptr;
     double last_kernel_ms = 0.0;
     uint64_t last_launches = 0;
@@ -266,6 +269,23 @@ Batch make_batch(int first_t, int n_t, unsigned long long chunks,
 
 constexpr unsigned long long kMaxTilesPerLaunch = 1ULL << 22;
 
+// Short lines go to the direct kernel: enough denominators per tile to give
+// every wave of the workgroup a full 64-pair slice, at most
+// kDirectMaxDenominators.
+Batch make_direct_batch(int first_t, int n_t, unsigned long long words) {
+    Batch bt;
+    bt.first_t = first_t;
+    bt.n_t = n_t;
+    bt.direct = true;
+    const unsigned long long want = 64ULL * rps::kDirectWaves;
+    int per = static_cast<int>(std::min<unsigned long long>(
+        rps::kDirectMaxDenominators, (want + words - 1) / words));
+    bt.tile_t = std::max(1, per);
+    bt.tiles = (static_cast<size_t>(n_t) + bt.tile_t - 1) / bt.tile_t;
+    bt.tile_words = words;
+    return bt;
+}
+
 }  // namespace
 
 extern "C" {
@@ -376,6 +396,12 @@ int rps_set_option(rps_ctx *ctx, const char *name, int64_t value) {
             return RPS_EINVAL;
         }
         ctx->variant = static_cast<int>(value);
+    } else if (key == "direct_words") {
+        if (value < 0 || value > static_cast<int64_t>(rps::kDirectMaxWords)) {
+            ctx->error = "direct_words must be in 0..768";
+            return RPS_EINVAL;
+        }
+        ctx->direct_words = static_cast<unsigned long long>(value);
     } else if (key == "min_tiles") {
         if (value < 1 || value > (1 << 22)) {
             ctx->error = "min_tiles must be in 1..4194304";
@@ -528,7 +554,7 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
 
         auto launch = [&](int si, const Batch &bt, bool resume) {
             Slot &s = ctx->slots[si];
-            ensure_cursors(s, bt.tiles * var->waves);
+            ensure_cursors(s, bt.tiles * std::max(var->waves, rps::kDirectWaves));
             rps::SieveParams prm = base;
             prm.first_t = bt.first_t;
             prm.n_t = bt.n_t;
@@ -541,7 +567,15 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
             prm.cursors = s.d_cursors;
             RPS_HIP(rps::launch_reset(s.d_state, ctx->compute));
             RPS_HIP(hipEventRecord(s.k_start, ctx->compute));
-            RPS_HIP(var->launch(prm, static_cast<unsigned>(bt.tiles), ctx->compute));
+            if (bt.direct) {
+                prm.tile_t = bt.tile_t;
+                prm.magic_words = words == 1 ? 0xFFFFFFFFu
+                    : static_cast<unsigned>(0x100000000ULL / words);
+                RPS_HIP(rps::launch_direct(prm, static_cast<unsigned>(bt.tiles),
+                                           ctx->compute));
+            } else {
+                RPS_HIP(var->launch(prm, static_cast<unsigned>(bt.tiles), ctx->compute));
+            }
             RPS_HIP(hipEventRecord(s.k_end, ctx->compute));
             RPS_HIP(hipMemcpyAsync(s.h_state, s.d_state, sizeof(rps::DeviceState),
                                    hipMemcpyDeviceToHost, ctx->compute));
@@ -557,6 +591,11 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
                 return false;
             }
             long long n = std::min<long long>(ctx->batch_size, last_t - next_t + 1);
+            if (words <= ctx->direct_words) {
+                bt = make_direct_batch(static_cast<int>(next_t), static_cast<int>(n), words);
+                next_t += n;
+                return true;
+            }
             unsigned long long chunks = line_chunks(ctx, n, words, stride, var->waves);
             if (chunks > kMaxTilesPerLaunch) {
                 throw ApiError{RPS_EINVAL, "maximum height exceeds GPU limit"};

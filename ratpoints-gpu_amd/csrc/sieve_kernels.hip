@@ -140,6 +140,60 @@ __global__ void reset_state_kernel(DeviceState *state) {
     state->pad = 0;
 }
 
+// Wave-level survivor emission.  Lane l holds U masks; mask u covers the 64
+// numerators of word first_word + 64 u on denominator b.  A prefix sum of
+// per-lane counts gives every lane its slots under one atomic reservation per
+// wave, all or nothing: when the reservation does not fit, first_fail records
+// its start, nothing is written and false is returned so the wave can stop
+// and resume at the same iteration after the host drains the buffer.
+template <int U>
+__device__ __forceinline__ bool emit_survivors(const SieveParams &P,
+                                               const uint64_t (&m)[U],
+                                               uint64_t first_word, int b,
+                                               int lane) {
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        cnt += __popcll(m[u]);
+    }
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t v = __shfl_up(incl, d, 64);
+        if (lane >= d) {
+            incl += v;
+        }
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    unsigned long long start = 0;
+    if (lane == 0) {
+        start = atomicAdd(&P.state->count, static_cast<unsigned long long>(total));
+    }
+    start = __shfl(start, 0, 64);
+    if (start + total > P.capacity) {
+        if (lane == 0) {
+            atomicMin(&P.state->first_fail, start);
+            atomicOr(&P.state->full, 1u);
+        }
+        return false;
+    }
+    unsigned long long pos = start + incl - cnt;
+    const uint64_t neg_a = static_cast<uint64_t>(-P.bound);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        uint64_t mm = m[u];
+        const uint64_t a0 = neg_a + 64 * (first_word + 64 * u);
+        while (mm) {
+            const int j = __ffsll(static_cast<long long>(mm)) - 1;
+            mm &= mm - 1;
+            P.out_num[pos] = static_cast<long long>(a0 + j);
+            P.out_den[pos] = b;
+            ++pos;
+        }
+    }
+    return true;
+}
+
 // The sieve.  Template parameters:
 //   U      64-bit words per lane per iteration (a wave covers 64 U words)
 //   FAST   selected primes ANDed densely for every word; the other
@@ -289,7 +343,6 @@ __global__ __launch_bounds__(WAVES * 64) void sieve_kernel(
     }
 
     const uint64_t last_word = P.words - 1;
-    const uint64_t neg_a = static_cast<uint64_t>(-P.bound);
     bool stopped = false;
     uint64_t W = ws;
     for (; W < we; W += kStride) {
@@ -367,56 +420,135 @@ __global__ __launch_bounds__(WAVES * 64) void sieve_kernel(
                 }
             }
         }
-        uint32_t cnt = 0;
+        uint64_t live = 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            cnt += __popcll(m[u]);
+            live |= m[u];
         }
-        if (!__ballot(cnt != 0)) {
+        if (!__ballot(live != 0)) {
             continue;
         }
-        // Wave prefix sum of survivor counts, one reservation per wave.  The
-        // iteration is committed whole or not at all, so a failed wave can
-        // resume at W after the host has drained the buffer.
-        uint32_t incl = cnt;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t v = __shfl_up(incl, d, 64);
-            if (lane >= d) {
-                incl += v;
-            }
-        }
-        const uint32_t total = __shfl(incl, 63, 64);
-        unsigned long long start = 0;
-        if (lane == 0) {
-            start = atomicAdd(&P.state->count,
-                              static_cast<unsigned long long>(total));
-        }
-        start = __shfl(start, 0, 64);
-        if (start + total > P.capacity) {
-            if (lane == 0) {
-                atomicMin(&P.state->first_fail, start);
-                atomicOr(&P.state->full, 1u);
-            }
+        if (!emit_survivors<U>(P, m, W + lane, b, lane)) {
             stopped = true;
             break;
-        }
-        unsigned long long pos = start + incl - cnt;
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            uint64_t mm = m[u];
-            const uint64_t a0 = neg_a + 64 * (W + lane + 64 * u);
-            while (mm) {
-                const int j = __ffsll(static_cast<long long>(mm)) - 1;
-                mm &= mm - 1;
-                P.out_num[pos] = static_cast<long long>(a0 + j);
-                P.out_den[pos] = b;
-                ++pos;
-            }
         }
     }
     if (lane == 0) {
         *cursor = stopped ? W : we;
+    }
+}
+
+// Short lines (few words per denominator): no row expansion.  A tile is a
+// block of up to NB denominators; the workgroup stages their residue strings
+// and prime parameters in LDS, and each lane takes one (denominator, word)
+// pair and ANDs the window of every selected prime directly, one Barrett
+// reduction and one funnel shift per prime, with a wave-uniform early exit
+// once every lane's word is empty.
+template <int WAVES, int NB>
+__global__ __launch_bounds__(WAVES * 64) void direct_kernel(
+    const SieveParams P) {
+    constexpr int kThreads = WAVES * 64;
+    __shared__ uint64_t cext[NB * kSelected * 10];
+    __shared__ uint32_t prm[NB * kSelected * 2];  // {p | c << 16, magic}
+    __shared__ int s_live;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = static_cast<int>(uniform(tid >> 6));
+    const int tile = blockIdx.x;
+    const int t0 = P.first_t + tile * P.tile_t;
+    const int nb = P.n_t - tile * P.tile_t < P.tile_t ? P.n_t - tile * P.tile_t : P.tile_t;
+    const uint32_t words = static_cast<uint32_t>(P.words);
+    const uint32_t total = static_cast<uint32_t>(nb) * words;
+    const uint32_t per = ((total + 64u * WAVES - 1) / (64u * WAVES)) * 64u;
+    uint32_t qs = per * wave;
+    const uint32_t qe = qs + per < total ? qs + per : total;
+    if (qs > qe) {
+        qs = qe;
+    }
+    unsigned long long *cursor =
+        P.cursors + static_cast<size_t>(tile) * WAVES + wave;
+    if (P.resume) {
+        qs = static_cast<uint32_t>(*cursor);
+    }
+    if (tid == 0) {
+        s_live = 0;
+    }
+    __syncthreads();
+    if (lane == 0 && qs < qe) {
+        atomicOr(&s_live, 1);
+    }
+    __syncthreads();
+    if (!s_live) {
+        return;
+    }
+    // Prime selection per denominator (src/sieve_plan.cpp:75-86), one wave
+    // per denominator; the lane that ranks a prime stages its string.
+    for (int bl = wave; bl < nb; bl += WAVES) {
+        const long long t = static_cast<long long>(t0) + bl;
+        const uint32_t b = static_cast<uint32_t>(P.square ? t * t : t);
+        const bool cand = lane < kCandidates;
+        const uint32_t p = cand ? static_cast<uint32_t>(P.primes[lane]) : 1u;
+        const uint32_t u = cand ? b % p : 0u;
+        const bool ok = cand && u != 0;
+        const unsigned long long mask = __ballot(ok);
+        if (ok) {
+            const int rank = __popcll(mask >> (lane + 1));
+            if (rank < kSelected) {
+                const int slot = bl * kSelected + rank;
+                prm[2 * slot] = p | (static_cast<uint32_t>(P.neg_bound_mod[lane]) << 16);
+                prm[2 * slot + 1] = static_cast<uint32_t>(0x100000000ULL / p);
+                const unsigned long long *src =
+                    P.resid + (static_cast<size_t>(lane) * kMaxPrime + u) * kResidWords;
+#pragma unroll
+                for (int w = 0; w < kResidWords; ++w) {
+                    cext[slot * 10 + w] = src[w];
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    bool stopped = false;
+    uint32_t Q = qs;
+    for (; Q < qe; Q += 64) {
+        const uint32_t q = Q + lane;
+        const bool valid = q < qe;
+        uint32_t bl = __umulhi(q, P.magic_words);
+        uint32_t w = q - bl * words;
+        if (w >= words) {
+            w -= words;
+            ++bl;
+        }
+        if (!valid) {
+            bl = 0;
+            w = 0;
+        }
+        uint64_t m[1];
+        m[0] = !valid ? 0ULL : (w == words - 1 ? P.last_mask : ~0ULL);
+        const uint32_t *pp = prm + bl * kSelected * 2;
+        const uint64_t *cx = cext + bl * kSelected * 10;
+        for (int s = 0; s < kSelected; ++s) {
+            if (s >= 8 && (s & 3) == 0 && !__ballot(m[0] != 0)) {
+                break;
+            }
+            const uint32_t pc = pp[2 * s];
+            const uint32_t p = pc & 0xffffu;
+            const uint32_t o = mod_small((pc >> 16) + 64u * w, p, pp[2 * s + 1]);
+            m[0] &= window64(cx + s * 10, o);
+        }
+        if (!__ballot(m[0] != 0)) {
+            continue;
+        }
+        const long long t = static_cast<long long>(t0) + bl;
+        const int b = static_cast<int>(P.square ? t * t : t);
+        if (!emit_survivors<1>(P, m, w, b, lane)) {
+            stopped = true;
+            break;
+        }
+    }
+    if (lane == 0) {
+        *cursor = stopped ? Q : qe;
     }
 }
 
@@ -465,6 +597,13 @@ hipError_t launch_resid(const int *primes, const unsigned long long *chi,
                         unsigned long long *resid, hipStream_t stream) {
     hipLaunchKernelGGL(resid_kernel, dim3(kCandidates, kMaxPrime / 4),
                        dim3(256), 0, stream, primes, chi, resid);
+    return hipGetLastError();
+}
+
+hipError_t launch_direct(const SieveParams &params, unsigned grid,
+                         hipStream_t stream) {
+    hipLaunchKernelGGL((direct_kernel<kDirectWaves, kDirectMaxDenominators>), dim3(grid),
+                       dim3(kDirectWaves * 64), 0, stream, params);
     return hipGetLastError();
 }
 

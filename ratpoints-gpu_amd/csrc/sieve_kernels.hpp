@@ -27,6 +27,14 @@ hipError_t launch_resid(const int *primes, const unsigned long long *chi,
                         unsigned long long *resid, hipStream_t stream);
 hipError_t launch_reset(DeviceState *state, hipStream_t stream);
 
+// Short-line kernel: tiles of up to kDirectMaxDenominators whole lines of at
+// most kDirectMaxWords words.
+constexpr int kDirectWaves = 8;
+constexpr int kDirectMaxDenominators = 16;
+constexpr unsigned kDirectMaxWords = 768;
+hipError_t launch_direct(const SieveParams &params, unsigned grid,
+                         hipStream_t stream);
+
 }  // namespace rps
 
 #endif

@@ -56,6 +56,8 @@ struct SieveParams {
     int tiles_per_t;                 // chunks per denominator line
     int square;                      // b = t^2
     int resume;                      // start every wave at its cursor
+    int tile_t;                      // denominators per short-line tile
+    unsigned magic_words;            // floor(2^32 / words), short lines
     int pad;
     int primes[kCandidates];         // ascending
     int neg_bound_mod[kCandidates];  // (-A) mod p

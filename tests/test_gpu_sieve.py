@@ -238,3 +238,49 @@ def test_curve_cache_switch(fresh_gpu, oracle_lib):
     for coeffs in (c1, c2, c1):
         res = fresh_gpu.sieve(coeffs, 4000, 1, 40)
         assert_same(res, oracle_lib.sieve(coeffs, 4000, 1, 40, method="words"))
+
+
+@pytest.mark.parametrize("bound", [1, 31, 32, 100, 1000, 9000, 24543])
+def test_short_lines_both_modes_match_oracle(fresh_gpu, oracle_lib, bound):
+    # Lines of at most 768 words go to the short-line kernel; the same input
+    # through the row kernel (direct_words = 0) and the oracle must agree.
+    coeffs, _ = curves.partially_dense(random.Random(bound), 6, 6)
+    first, last = 100, 100 + max(40, 40000 // (2 * bound + 1))
+    announce(case="short-lines", A=bound, b=f"{first}..{last}")
+    expected = oracle_lib.sieve(coeffs, bound, first, last, method="words")
+    res = fresh_gpu.sieve(coeffs, bound, first, last)
+    assert_same(res, expected)
+    fresh_gpu.set_option("direct_words", 0)
+    assert_same(fresh_gpu.sieve(coeffs, bound, first, last), expected)
+
+
+def test_short_lines_many_denominators(fresh_gpu, oracle_lib, workloads):
+    # Config 1 in full: the planted sextic on [-1000, 1000] x [1, 1000].
+    coeffs, points = workloads.config1_curve()
+    announce(case="config1")
+    res = fresh_gpu.sieve(coeffs, 1000, 1, 1000)
+    assert_same(res, oracle_lib.sieve(coeffs, 1000, 1, 1000, method="words"))
+    got = set(zip(res.numerators.tolist(), res.denominators.tolist()))
+    assert all(p in got for p in points)
+
+
+def test_short_lines_streaming_resume(fresh_gpu, oracle_lib):
+    # Dense 1 + P x on short lines over many tiles with a small buffer.  The
+    # line is fully dense only where the 32 selected primes are P's; the
+    # denominators 331..397 in range are candidate primes themselves, bring
+    # in 313 and lose about half their sites, so the oracle is the reference.
+    announce(case="short-dense-resume")
+    fresh_gpu.set_option("survivor_capacity", 16384)
+    coeffs = ["1", str(curves.dense_product())]
+    res = fresh_gpu.sieve(coeffs, 300, 1, 400)
+    expected = oracle_lib.sieve(coeffs, 300, 1, 400, method="words")
+    assert expected[0] > 230_000
+    assert res.stats["launches"] >= expected[0] // 16384
+    assert_same(res, expected)
+
+
+def test_short_lines_square_mode(fresh_gpu, oracle_lib):
+    coeffs, _ = curves.partially_dense(random.Random(31), 5, 4)
+    announce(case="short-square")
+    res = fresh_gpu.sieve(coeffs, 700, 46000, 46340, square=True)
+    assert_same(res, oracle_lib.sieve(coeffs, 700, 46000, 46340, square=True, method="words"))
