@@ -75,6 +75,19 @@ def cpu_baseline(coeffs, seconds_target=15.0):
     }
 
 
+def measured_traffic(variant):
+    """HBM bytes per sieve launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (scaled by a copy kernel of known size), when they were
+    taken for this kernel variant; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    return data.get("hbm_bytes_per_launch") if data.get("kernel") == variant else None
+
+
 def verify_points(coeffs, nums, dens):
     """Exact stage on the survivors (gcd = 1, F(a, b) a perfect square)."""
     n = len(coeffs) - 1
@@ -178,7 +191,7 @@ def main():
             "peak": LDS_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / LDS_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(variant),
             "kernel": variant,
             "kernel_ms_per_launch": round(per_launch_ms, 3),
             "bytes_per_site": LDS_BYTES_PER_SITE,

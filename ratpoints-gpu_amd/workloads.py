@@ -75,6 +75,43 @@ def is_squarefree(coeffs):
     return len(f) == 1
 
 
+def _poly_mod(p, m):
+    return [c % m for c in p]
+
+
+def _trim_mod(p):
+    while p and p[-1] == 0:
+        p.pop()
+    return p
+
+
+def _gcd_degree_mod(f, g, m):
+    """Degree of gcd(f, g) over GF(m), m prime."""
+    f = _trim_mod(_poly_mod(f, m))
+    g = _trim_mod(_poly_mod(g, m))
+    while g:
+        inv = pow(g[-1], m - 2, m)
+        r = list(f)
+        while len(r) >= len(g) and r:
+            shift = len(r) - len(g)
+            factor = r[-1] * inv % m
+            for i in range(len(g)):
+                r[i + shift] = (r[i + shift] - factor * g[i]) % m
+            _trim_mod(r)
+        f, g = g, r
+    return len(f) - 1
+
+
+def squarefree_fast(coeffs, modulus=(1 << 61) - 1):
+    """Squarefree over Q.  If f is squarefree modulo a prime that does not
+    divide its leading coefficient, it is squarefree over Q; otherwise the
+    exact test decides."""
+    if coeffs[-1] % modulus and _gcd_degree_mod(
+            coeffs, [coeffs[i] * i for i in range(1, len(coeffs))], modulus) == 0:
+        return True
+    return is_squarefree(coeffs)
+
+
 def planted_curve(seed, q_degree, q_bound, point_bound, n_points, r_bound=100,
                   degree=None):
     """Draws f = q^2 + r * prod (b_i x - a_i) with coprime planted points.
@@ -130,3 +167,67 @@ RECORD_CURVE = [247747600, -985905640, 567207969, 2396040466, 52485681,
 
 def sites(bound, first, last):
     return (2 * bound + 1) * (last - first + 1)
+
+
+def _log_uniform_int(rng, lo, hi):
+    """Integer log-uniform in [lo, hi] from 53 random bits."""
+    u = (rng.next() >> 11) / float(1 << 53)
+    return max(lo, min(hi, int(round(math.exp(math.log(lo) + u * (math.log(hi) - math.log(lo)))))))
+
+
+def batched_curve(rng, degree, bits, n_points, bound, den_bound):
+    """One curve of the batched workload.  Even degree d: f = q^2 +
+    r prod_i (b_i x - a_i) with deg q = d/2 and the n_points planted factors.
+    Odd degree d: q has degree (d-1)/2 and the product is padded with random
+    linear factors to degree d, so that deg f = d (q^2 alone would have degree
+    d + 1).  q's coefficients are drawn with bits/2 bits so q^2 has about
+    `bits`-bit coefficients.  Planted points lie inside the search box; for
+    odd degree at most d of them fit the construction, so n_points is capped
+    at d."""
+    if degree % 2 == 1:
+        n_points = min(n_points, degree)
+    half = max(1, bits // 2)
+    while True:
+        q_deg = degree // 2 if degree % 2 == 0 else (degree - 1) // 2
+        q = [rng.uniform(-(1 << half), 1 << half) for _ in range(q_deg + 1)]
+        if q[-1] == 0:
+            continue
+        r = 0
+        while r == 0:
+            r = rng.uniform(-100, 100)
+        points = []
+        while len(points) < n_points:
+            a = rng.uniform(-bound, bound)
+            b = rng.uniform(1, den_bound)
+            if math.gcd(a, b) == 1 and (a, b) not in points:
+                points.append((a, b))
+        factors = [[-a, b] for a, b in points]
+        if degree % 2 == 1:
+            while len(factors) < degree:
+                factors.append([rng.uniform(-1000, 1000), rng.uniform(1, 1000)])
+        prod = [1]
+        for fac in factors:
+            prod = poly_mul(prod, fac)
+        f = poly_add(poly_mul(q, q), [r * c for c in prod])
+        _trim(f)
+        if len(f) - 1 == degree and squarefree_fast(f):
+            return f, points
+
+
+def config3_curves(seed=3, count=4096):
+    """Batched form: degrees uniform over {3, 4, 5, 6, 8, 10}, coefficient
+    size uniform in 8..256 bits, 0-4 planted points, numerator and
+    denominator bounds independently log-uniform in [1e3, 1e6].
+    Returns a list of dicts {coefficients, bound, first, last, points}."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        degree = (3, 4, 5, 6, 8, 10)[rng.uniform(0, 5)]
+        bits = rng.uniform(8, 256)
+        n_points = rng.uniform(0, 4)
+        bound = _log_uniform_int(rng, 1000, 1_000_000)
+        den = _log_uniform_int(rng, 1000, 1_000_000)
+        coeffs, points = batched_curve(rng, degree, bits, n_points, bound, den)
+        out.append({"coefficients": coeffs, "bound": bound, "first": 1, "last": den,
+                    "points": points, "degree": degree})
+    return out
