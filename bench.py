@@ -88,6 +88,64 @@ def measured_traffic(variant):
     return data.get("hbm_bytes_per_launch") if data.get("kernel") == variant else None
 
 
+def batched_leg(pkg, wl):
+    """Config 3: 4,096 curves (degrees 3-10, 8-256-bit coefficients, 0-4
+    planted points, A and B log-uniform in [1e3, 1e6]) in one persistent
+    rps_sieve_many launch, against looping rps_sieve curve by curve on the
+    same GPU.  Both must give the same survivor set per curve."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # digest helper only
+    curves = wl.config3_curves()
+    sites = sum((2 * c["bound"] + 1) * (c["last"] - c["first"] + 1) for c in curves)
+    ctx = pkg.SieveContext(0)
+    digests = [(0, 0)] * len(curves)
+    counts = [0] * len(curves)
+
+    def many_sink(req, num, den):
+        for r in np.unique(req):
+            sel = req == r
+            digests[r] = oracle.add_digests(digests[r], oracle.site_digest(num[sel], den[sel]))
+            counts[r] += int(sel.sum())
+        return False
+
+    ctx.run_many(curves[:64], lambda q, n, d: False)  # warm tables and code
+    print(f"bench: batched {len(curves)} curves ({sites:.3e} sites)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    rc, stats = ctx.run_many(curves, many_sink)
+    many_wall = time.perf_counter() - t0
+    many_kernel_ms, many_launches, _ = ctx.kernel_timing()
+    loop_counts = [0] * len(curves)
+    loop_digests = [(0, 0)] * len(curves)
+    print("bench: plain loop over the same curves", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for i, c in enumerate(curves):
+        def sink(num, den, i=i):
+            loop_digests[i] = oracle.add_digests(loop_digests[i], oracle.site_digest(num, den))
+            loop_counts[i] += len(num)
+            return False
+        ctx.run(c["coefficients"], c["bound"], c["first"], c["last"], False, sink)
+    loop_wall = time.perf_counter() - t0
+    ctx.close()
+    same = counts == loop_counts and digests == loop_digests
+    return {
+        "workload": "config 3: 4096 planted curves, seed 3",
+        "curves": len(curves),
+        "sites": sites,
+        "batched_curves_per_s": round(len(curves) / many_wall, 1),
+        "batched_wall_s": round(many_wall, 3),
+        "batched_gsites": round(sites / many_wall / 1e9, 1),
+        "batched_kernel_ms": round(many_kernel_ms, 1),
+        "batched_launches": many_launches,
+        "requeued": stats["requeued"],
+        "plain_loop_curves_per_s": round(len(curves) / loop_wall, 1),
+        "plain_loop_wall_s": round(loop_wall, 3),
+        "speedup": round(loop_wall / many_wall, 3),
+        "survivors": sum(counts),
+        "identical_to_plain_loop": same,
+    }
+
+
 def verify_points(coeffs, nums, dens):
     """Exact stage on the survivors (gcd = 1, F(a, b) a perfect square)."""
     n = len(coeffs) - 1
@@ -111,6 +169,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched (4,096-curve) leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; one MI355X covers the whole "
@@ -206,11 +266,15 @@ def main():
     }
     if missing:
         result["error"] = f"planted points missing: {missing}"
+    if not args.no_batched:
+        result["batched"] = batched_leg(pkg, wl)
+        if not result["batched"]["identical_to_plain_loop"]:
+            result["error"] = "batched survivors differ from the plain loop"
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(coeffs)
     print(json.dumps(result), flush=True)
-    if missing:
+    if "error" in result:
         sys.exit(1)
 
 

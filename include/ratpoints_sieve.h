@@ -85,7 +85,9 @@ typedef struct rps_stats {
     double sieve_wall_ms;
     double initial_mask_bytes;
     uint64_t launches;        /* sieve kernel launches, resumes included  */
-    uint64_t batches;         /* denominator batches                      */
+    uint64_t batches;         /* denominator batches (requests, for many) */
+    uint64_t requeued;        /* requests searched again on the plain
+                                 path after a survivor-buffer overflow    */
 } rps_stats;
 
 /* Opens a context on HIP device `device` (0-based among visible devices).
@@ -123,6 +125,27 @@ typedef int (*rps_batch_done)(void *user, int32_t first_t, int32_t last_t,
 /* rps_sieve with a batch-completion callback (`done` may be NULL). */
 int rps_sieve_batched(rps_ctx *ctx, const rps_request *request, rps_sink sink,
                       rps_batch_done done, void *user, rps_stats *stats);
+
+/* Survivor sink of a many-curve request: request[i] is the index into the
+ * request array of survivor i.  Same validity, order and cancel rules as
+ * rps_sink. */
+typedef int (*rps_multi_sink)(void *user, const int32_t *request,
+                              const int64_t *numerators,
+                              const int32_t *denominators, size_t count);
+
+/* Sieves many requests (curves with their own coefficients, bounds, ranges
+ * and modes) in one persistent launch: workgroups pull (curve, denominator
+ * block) and (curve, line chunk) tiles of all requests from a device work
+ * counter, so short searches share the GPU instead of each paying a launch
+ * and a synchronisation.  Each request's survivor set equals what rps_sieve
+ * gives for it alone.  A request whose survivors overflow the buffer during
+ * the shared launch is searched again alone (stats->requeued counts them).
+ * No counterpart in the reference, which searches one curve per process
+ * (src/main.cpp:48-85); it is the batched form of the same
+ * run_modular_sieve contract (include/sieve.hpp:43-47). */
+int rps_sieve_many(rps_ctx *ctx, const rps_request *requests,
+                   size_t n_requests, rps_multi_sink sink, void *user,
+                   rps_stats *stats);
 
 /* Tuning and test knobs; returns RPS_EINVAL for an unknown name or a value
  * out of range.  No counterpart in the reference, whose knobs were

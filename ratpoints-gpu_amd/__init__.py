@@ -56,6 +56,7 @@ class RpsStats(ctypes.Structure):
         ("initial_mask_bytes", ctypes.c_double),
         ("launches", ctypes.c_uint64),
         ("batches", ctypes.c_uint64),
+        ("requeued", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -64,6 +65,9 @@ class RpsStats(ctypes.Structure):
 
 SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
                         ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t)
+MULTI_SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
+                              ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32),
+                              ctypes.c_size_t)
 
 _LIB = None
 
@@ -92,6 +96,9 @@ def load_library(path=LIB_PATH):
     lib.rps_sieve.restype = ctypes.c_int
     lib.rps_sieve.argtypes = [ctypes.c_void_p, ctypes.POINTER(RpsRequest), SINK,
                               ctypes.c_void_p, ctypes.POINTER(RpsStats)]
+    lib.rps_sieve_many.restype = ctypes.c_int
+    lib.rps_sieve_many.argtypes = [ctypes.c_void_p, ctypes.POINTER(RpsRequest), ctypes.c_size_t,
+                                   MULTI_SINK, ctypes.c_void_p, ctypes.POINTER(RpsStats)]
     lib.rps_set_option.restype = ctypes.c_int
     lib.rps_set_option.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]
     lib.rps_variant_count.restype = ctypes.c_int
@@ -197,6 +204,69 @@ class SieveContext:
         if rc not in (RPS_OK, RPS_ECANCELLED):
             raise SieveError(rc, self.error())
         return rc, stats.as_dict()
+
+    def run_many(self, requests, sink=None):
+        """Runs rps_sieve_many.  ``requests`` is a list of dicts with keys
+        coefficients, bound, first, last and optional square.  ``sink(req,
+        num, den)`` gets numpy views valid only during the call; return True
+        to cancel.  Returns (status, stats)."""
+        keep = []
+        reqs = (RpsRequest * len(requests))()
+        for i, r in enumerate(requests):
+            coeffs = [str(c) for c in r["coefficients"]]
+            arr = (ctypes.c_char_p * len(coeffs))()
+            arr[:] = [c.encode() for c in coeffs]
+            keep.append(arr)
+            reqs[i] = RpsRequest(arr, len(coeffs), int(r["bound"]), int(r["first"]),
+                                 int(r["last"]), int(bool(r.get("square", False))))
+        stats = RpsStats()
+        error = []
+
+        def trampoline(_user, req_ptr, num_ptr, den_ptr, n):
+            try:
+                req = np.ctypeslib.as_array(req_ptr, shape=(n,))
+                num = np.ctypeslib.as_array(num_ptr, shape=(n,))
+                den = np.ctypeslib.as_array(den_ptr, shape=(n,))
+                return 1 if (sink(req, num, den) if sink else False) else 0
+            except BaseException as exc:
+                error.append(exc)
+                return 1
+
+        cb = MULTI_SINK(trampoline)
+        rc = self.lib.rps_sieve_many(self.handle, reqs, len(requests), cb, None,
+                                     ctypes.byref(stats))
+        if error:
+            raise error[0]
+        if rc not in (RPS_OK, RPS_ECANCELLED):
+            raise SieveError(rc, self.error())
+        return rc, stats.as_dict()
+
+    def sieve_many(self, requests):
+        """Collects every survivor of every request: returns (results, stats)
+        with results[i] = (num, den) sorted by (b, a)."""
+        parts = {}
+
+        def sink(req, num, den):
+            parts.setdefault("req", []).append(req.copy())
+            parts.setdefault("num", []).append(num.copy())
+            parts.setdefault("den", []).append(den.copy())
+            return False
+
+        rc, stats = self.run_many(requests, sink)
+        if parts:
+            req = np.concatenate(parts["req"])
+            num = np.concatenate(parts["num"])
+            den = np.concatenate(parts["den"])
+        else:
+            req = np.zeros(0, dtype=np.int32)
+            num = np.zeros(0, dtype=np.int64)
+            den = np.zeros(0, dtype=np.int32)
+        order = np.lexsort((num, den, req))
+        req, num, den = req[order], num[order], den[order]
+        bounds = np.searchsorted(req, np.arange(len(requests) + 1))
+        results = [(num[bounds[i]:bounds[i + 1]], den[bounds[i]:bounds[i + 1]])
+                   for i in range(len(requests))]
+        return results, stats
 
     def sieve(self, coefficients, bound, first, last, square=False):
         """Collects every survivor; returns SieveResult with arrays sorted by

@@ -1,5 +1,6 @@
 // C ABI of the gfx950 modular square sieve: context, per-curve device
-// tables, denominator batching and the double-buffered survivor stream.
+// tables, denominator batching, the double-buffered survivor stream and the
+// persistent many-curve launch.
 //
 // Host orchestration replaces run_modular_sieve (src/sieve.cu:503-580 in the
 // reference): the reference rebuilds its 24 MB basis table and reallocates
@@ -14,6 +15,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -67,15 +69,20 @@ bool is_decimal(const char *text) {
     return true;
 }
 
-// Signed decimal text mod p, in [0, p).
+// Signed decimal text mod p, in [0, p), nine digits per step.
 int decimal_mod(const char *text, int p) {
-    bool negative = *text == '-';
+    const bool negative = *text == '-';
     if (*text == '-' || *text == '+') {
         ++text;
     }
     long long v = 0;
-    for (; *text; ++text) {
-        v = (10 * v + (*text - '0')) % p;
+    while (*text) {
+        long long chunk = 0, scale = 1;
+        for (int k = 0; k < 9 && *text; ++k, ++text) {
+            chunk = 10 * chunk + (*text - '0');
+            scale *= 10;
+        }
+        v = (v * (scale % p) + chunk) % p;
     }
     return static_cast<int>(negative && v ? p - v : v);
 }
@@ -115,20 +122,45 @@ struct Slot {
     hipEvent_t k_start = nullptr, k_end = nullptr, ready = nullptr;
 };
 
+// One launch of the plain path: a batch of denominators of one curve.
 struct Batch {
-    int first_t = 0;
-    int n_t = 0;
-    int tiles_per_t = 1;
-    unsigned long long tile_words = 0;
+    rps::LineDesc line{};
     size_t tiles = 0;
-    bool direct = false;  // short lines: tiles of tile_t whole lines
-    int tile_t = 1;
+};
+
+// Device buffer that only grows.
+template <typename T>
+struct Grow {
+    T *ptr = nullptr;
+    size_t size = 0;
+
+    void ensure(size_t n) {
+        if (n <= size) {
+            return;
+        }
+        if (ptr) {
+            RPS_HIP(hipFree(ptr));
+            ptr = nullptr;
+            size = 0;
+        }
+        RPS_HIP(hipMalloc(&ptr, n * sizeof(T)));
+        size = n;
+    }
+
+    void release() {
+        if (ptr) {
+            (void)hipFree(ptr);
+        }
+        ptr = nullptr;
+        size = 0;
+    }
 };
 
 }  // namespace
 
 struct rps_ctx {
     int device = 0;
+    int cu_count = 256;
     hipStream_t compute = nullptr;
     hipStream_t copy = nullptr;
     std::string error;
@@ -137,6 +169,8 @@ struct rps_ctx {
     int *d_coef_res = nullptr;
     unsigned long long *d_chi = nullptr;
     unsigned long long *d_resid = nullptr;
+    int *d_negmod = nullptr;
+    int *h_negmod = nullptr;  // pinned
     std::vector<std::string> curve_key;
     bool curve_ready = false;
     Slot slots[2];
@@ -144,6 +178,7 @@ struct rps_ctx {
     size_t allocated_capacity = 0;
     long long *h_num = nullptr;  // pinned staging
     int *h_den = nullptr;
+    int *h_req = nullptr;
     size_t stage_capacity = size_t(1) << 20;
     int batch_size = 1 << 16;
     int variant = 0;
@@ -153,6 +188,16 @@ struct rps_ctx {
     double last_kernel_ms = 0.0;
     uint64_t last_launches = 0;
     double last_sites = 0.0;
+    // Many-curve launches.
+    Grow<int> m_coef;
+    Grow<unsigned long long> m_chi;
+    Grow<unsigned long long> m_resid;
+    Grow<int> m_negmod;
+    Grow<rps::CurveDesc> m_curves;
+    Grow<unsigned long long> m_tile_start;
+    Grow<int> m_failed;
+    Grow<unsigned long long> m_head;
+    Grow<int> m_req;
 };
 
 namespace {
@@ -201,6 +246,57 @@ void ensure_cursors(Slot &s, size_t entries) {
     s.cursor_capacity = n;
 }
 
+void validate(const rps_request *req) {
+    if (!req) {
+        throw ApiError{RPS_EINVAL, "null request"};
+    }
+    if (req->n_coefficients < 2 || req->n_coefficients > rps::kMaxCoefficients
+        || !req->coefficients) {
+        throw ApiError{RPS_EINVAL, "supported polynomial degrees are 1 through 100"};
+    }
+    for (int k = 0; k < req->n_coefficients; ++k) {
+        if (!is_decimal(req->coefficients[k])) {
+            throw ApiError{RPS_EINVAL, "coefficients must be decimal integers"};
+        }
+    }
+    if (req->numerator_bound < 1 || req->numerator_bound > (LLONG_MAX - 32) / 2) {
+        throw ApiError{RPS_EINVAL, "invalid numerator bound"};
+    }
+    // src/sieve_plan.cpp:55-58
+    if (req->den_first < 1 || req->den_first > req->den_last
+        || (req->square_denominators && req->den_last > rps::kMaxParamT)) {
+        throw ApiError{RPS_EINVAL, "invalid denominator range"};
+    }
+}
+
+// Line geometry of a request: words per line, last-word mask, and the
+// fields of LineDesc that do not depend on the batch.
+rps::LineDesc line_geometry(const rps_request *req) {
+    rps::LineDesc L{};
+    const unsigned long long A = static_cast<unsigned long long>(req->numerator_bound);
+    L.bound = req->numerator_bound;
+    L.words = (2ULL * A + 64) / 64;
+    const unsigned long long valid_last = 2ULL * A - 64ULL * (L.words - 1) + 1;
+    L.last_mask = valid_last >= 64 ? ~0ULL : ((1ULL << valid_last) - 1);
+    L.square = req->square_denominators ? 1 : 0;
+    L.magic_words = L.words == 1 ? 0xFFFFFFFFu
+        : static_cast<unsigned>(0x100000000ULL / std::min<unsigned long long>(L.words, 0xFFFFFFFFULL));
+    L.tiles_per_t = 1;
+    L.tile_t = 1;
+    L.tile_words = L.words;
+    return L;
+}
+
+void coefficient_residues(const rps_request *req, const std::vector<int> &primes,
+                          int n_coef, int *out /* [36][n_coef] */) {
+    for (int i = 0; i < rps::kCandidates; ++i) {
+        for (int k = 0; k < n_coef; ++k) {
+            out[static_cast<size_t>(i) * n_coef + k] = k < req->n_coefficients
+                ? decimal_mod(req->coefficients[k], primes[i]) : 0;
+        }
+    }
+}
+
 void prepare_curve(rps_ctx *ctx, const rps_request *req, rps_stats *st) {
     std::vector<std::string> key(req->coefficients,
                                  req->coefficients + req->n_coefficients);
@@ -209,21 +305,13 @@ void prepare_curve(rps_ctx *ctx, const rps_request *req, rps_stats *st) {
     }
     const int n = req->n_coefficients;
     std::vector<int> res(static_cast<size_t>(rps::kCandidates) * n);
-    for (int i = 0; i < rps::kCandidates; ++i) {
-        for (int k = 0; k < n; ++k) {
-            res[static_cast<size_t>(i) * n + k] =
-                decimal_mod(req->coefficients[k], ctx->primes[i]);
-        }
-    }
+    coefficient_residues(req, ctx->primes, n, res.data());
     ctx->curve_ready = false;
-    RPS_HIP(hipMemcpyAsync(ctx->d_coef_res, res.data(),
-                           res.size() * sizeof(int), hipMemcpyHostToDevice,
-                           ctx->compute));
+    RPS_HIP(hipMemcpyAsync(ctx->d_coef_res, res.data(), res.size() * sizeof(int),
+                           hipMemcpyHostToDevice, ctx->compute));
     RPS_HIP(hipEventRecord(ctx->basis_start, ctx->compute));
-    RPS_HIP(rps::launch_chi(ctx->d_coef_res, n, ctx->d_primes, ctx->d_chi,
-                            ctx->compute));
-    RPS_HIP(rps::launch_resid(ctx->d_primes, ctx->d_chi, ctx->d_resid,
-                              ctx->compute));
+    RPS_HIP(rps::launch_chi(ctx->d_coef_res, n, 1, ctx->d_primes, ctx->d_chi, ctx->compute));
+    RPS_HIP(rps::launch_resid(ctx->d_primes, ctx->d_chi, 1, ctx->d_resid, ctx->compute));
     RPS_HIP(hipEventRecord(ctx->basis_end, ctx->compute));
     RPS_HIP(hipEventSynchronize(ctx->basis_end));
     float ms = 0.0f;
@@ -240,12 +328,11 @@ void prepare_curve(rps_ctx *ctx, const rps_request *req, rps_stats *st) {
 // always into chunks of at most 2^31 words (the kernel's local word offsets
 // are 32-bit).  A chunk keeps at least eight strides per wave so the per-tile
 // row build stays amortised.
-unsigned long long line_chunks(const rps_ctx *ctx, long long n_t,
-                               unsigned long long words, int stride,
-                               int waves) {
+unsigned long long line_chunks(long long min_tiles, long long n_t,
+                               unsigned long long words, int stride, int waves) {
     unsigned long long chunks = 1;
-    if (n_t < ctx->min_tiles) {
-        chunks = (static_cast<unsigned long long>(ctx->min_tiles) + n_t - 1) / n_t;
+    if (n_t < min_tiles) {
+        chunks = (static_cast<unsigned long long>(min_tiles) + n_t - 1) / n_t;
     }
     const unsigned long long min_words = 8ULL * stride * waves;
     chunks = std::min(chunks, std::max(1ULL, words / min_words));
@@ -253,37 +340,47 @@ unsigned long long line_chunks(const rps_ctx *ctx, long long n_t,
     return std::max(chunks, (words + limit - 1) / limit);
 }
 
-Batch make_batch(int first_t, int n_t, unsigned long long chunks,
-                 unsigned long long words, int stride) {
-    Batch bt;
-    bt.first_t = first_t;
-    bt.n_t = n_t;
-    unsigned long long tile = (words + chunks - 1) / chunks;
+// Fills the tiling fields of L for row tiles of `chunks` chunks per line.
+void row_tiling(rps::LineDesc &L, unsigned long long chunks, int stride) {
+    unsigned long long tile = (L.words + chunks - 1) / chunks;
     tile = (tile + stride - 1) / stride * stride;
-    chunks = (words + tile - 1) / tile;
-    bt.tile_words = tile;
-    bt.tiles_per_t = static_cast<int>(chunks);
-    bt.tiles = static_cast<size_t>(n_t) * chunks;
-    return bt;
+    L.direct = 0;
+    L.tile_words = tile;
+    L.tiles_per_t = static_cast<int>((L.words + tile - 1) / tile);
+}
+
+// Short lines: enough denominators per tile to give every wave of the
+// workgroup a full 64-pair slice, at most kDirectMaxDenominators.
+void direct_tiling(rps::LineDesc &L) {
+    const unsigned long long want = 64ULL * rps::kDirectWaves;
+    const int per = static_cast<int>(std::min<unsigned long long>(
+        rps::kDirectMaxDenominators, (want + L.words - 1) / L.words));
+    L.direct = 1;
+    L.tile_t = std::max(1, per);
+}
+
+size_t tile_count(const rps::LineDesc &L) {
+    if (L.direct) {
+        return (static_cast<size_t>(L.n_t) + L.tile_t - 1) / L.tile_t;
+    }
+    return static_cast<size_t>(L.n_t) * static_cast<size_t>(L.tiles_per_t);
 }
 
 constexpr unsigned long long kMaxTilesPerLaunch = 1ULL << 22;
 
-// Short lines go to the direct kernel: enough denominators per tile to give
-// every wave of the workgroup a full 64-pair slice, at most
-// kDirectMaxDenominators.
-Batch make_direct_batch(int first_t, int n_t, unsigned long long words) {
-    Batch bt;
-    bt.first_t = first_t;
-    bt.n_t = n_t;
-    bt.direct = true;
-    const unsigned long long want = 64ULL * rps::kDirectWaves;
-    int per = static_cast<int>(std::min<unsigned long long>(
-        rps::kDirectMaxDenominators, (want + words - 1) / words));
-    bt.tile_t = std::max(1, per);
-    bt.tiles = (static_cast<size_t>(n_t) + bt.tile_t - 1) / bt.tile_t;
-    bt.tile_words = words;
-    return bt;
+int fail(rps_ctx *ctx, const HipError &e) {
+    ctx->error = op_name(e.op) + ": " + hipGetErrorString(e.code);
+    (void)hipStreamSynchronize(ctx->compute);
+    (void)hipStreamSynchronize(ctx->copy);
+    ctx->curve_ready = false;
+    return RPS_EDEVICE;
+}
+
+int fail(rps_ctx *ctx, const ApiError &e) {
+    ctx->error = e.message;
+    (void)hipStreamSynchronize(ctx->compute);
+    (void)hipStreamSynchronize(ctx->copy);
+    return e.code;
 }
 
 }  // namespace
@@ -309,6 +406,9 @@ int rps_open(int device, rps_ctx **out) {
         }
         ctx->device = device;
         RPS_HIP(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        RPS_HIP(hipGetDeviceProperties(&prop, device));
+        ctx->cu_count = prop.multiProcessorCount;
         RPS_HIP(hipStreamCreateWithFlags(&ctx->compute, hipStreamNonBlocking));
         RPS_HIP(hipStreamCreateWithFlags(&ctx->copy, hipStreamNonBlocking));
         ctx->primes = candidate_primes();
@@ -319,9 +419,9 @@ int rps_open(int device, rps_ctx **out) {
                               * rps::kMaxCoefficients * sizeof(int)));
         RPS_HIP(hipMalloc(&ctx->d_chi, static_cast<size_t>(rps::kCandidates)
                               * rps::kChiWords * sizeof(unsigned long long)));
-        RPS_HIP(hipMalloc(&ctx->d_resid, static_cast<size_t>(rps::kCandidates)
-                              * rps::kMaxPrime * rps::kResidWords
-                              * sizeof(unsigned long long)));
+        RPS_HIP(hipMalloc(&ctx->d_resid, rps::kResidTableWords * sizeof(unsigned long long)));
+        RPS_HIP(hipMalloc(&ctx->d_negmod, rps::kCandidates * sizeof(int)));
+        RPS_HIP(hipHostMalloc(&ctx->h_negmod, rps::kCandidates * sizeof(int)));
         RPS_HIP(hipEventCreate(&ctx->basis_start));
         RPS_HIP(hipEventCreate(&ctx->basis_end));
         for (Slot &s : ctx->slots) {
@@ -333,6 +433,7 @@ int rps_open(int device, rps_ctx **out) {
         }
         RPS_HIP(hipHostMalloc(&ctx->h_num, ctx->stage_capacity * sizeof(long long)));
         RPS_HIP(hipHostMalloc(&ctx->h_den, ctx->stage_capacity * sizeof(int)));
+        RPS_HIP(hipHostMalloc(&ctx->h_req, ctx->stage_capacity * sizeof(int)));
     } catch (const HipError &e) {
         g_open_error = op_name(e.op) + ": " + hipGetErrorString(e.code);
         rps_close(ctx);
@@ -360,8 +461,20 @@ void rps_close(rps_ctx *ctx) {
     if (ctx->d_coef_res) (void)hipFree(ctx->d_coef_res);
     if (ctx->d_chi) (void)hipFree(ctx->d_chi);
     if (ctx->d_resid) (void)hipFree(ctx->d_resid);
+    if (ctx->d_negmod) (void)hipFree(ctx->d_negmod);
+    if (ctx->h_negmod) (void)hipHostFree(ctx->h_negmod);
     if (ctx->h_num) (void)hipHostFree(ctx->h_num);
     if (ctx->h_den) (void)hipHostFree(ctx->h_den);
+    if (ctx->h_req) (void)hipHostFree(ctx->h_req);
+    ctx->m_coef.release();
+    ctx->m_chi.release();
+    ctx->m_resid.release();
+    ctx->m_negmod.release();
+    ctx->m_curves.release();
+    ctx->m_tile_start.release();
+    ctx->m_failed.release();
+    ctx->m_head.release();
+    ctx->m_req.release();
     if (ctx->basis_start) (void)hipEventDestroy(ctx->basis_start);
     if (ctx->basis_end) (void)hipEventDestroy(ctx->basis_end);
     if (ctx->compute) (void)hipStreamDestroy(ctx->compute);
@@ -446,7 +559,7 @@ int rps_device_info(int device, char *name, size_t name_len, char *pci,
         return RPS_EDEVICE;
     }
     if (name && name_len) {
-        std::snprintf(name, name_len, "%s", prop.name);
+        std::snprintf(name, name_len, "%s", prop.name[0] ? prop.name : prop.gcnArchName);
     }
     if (pci && pci_len) {
         char bus[64] = {0};
@@ -492,59 +605,38 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
     ctx->last_sites = 0.0;
     const auto t_plan = Clock::now();
     try {
-        if (!req || !sink) {
-            throw ApiError{RPS_EINVAL, "null request or sink"};
+        if (!sink) {
+            throw ApiError{RPS_EINVAL, "null sink"};
         }
-        if (req->n_coefficients < 2
-            || req->n_coefficients > rps::kMaxCoefficients
-            || !req->coefficients) {
-            throw ApiError{RPS_EINVAL, "supported polynomial degrees are 1 through 100"};
-        }
-        for (int k = 0; k < req->n_coefficients; ++k) {
-            if (!is_decimal(req->coefficients[k])) {
-                throw ApiError{RPS_EINVAL, "coefficients must be decimal integers"};
-            }
-        }
-        if (req->numerator_bound < 1
-            || req->numerator_bound > (LLONG_MAX - 32) / 2) {
-            throw ApiError{RPS_EINVAL, "invalid numerator bound"};
-        }
-        // src/sieve_plan.cpp:55-58
-        if (req->den_first < 1 || req->den_first > req->den_last
-            || (req->square_denominators && req->den_last > rps::kMaxParamT)) {
-            throw ApiError{RPS_EINVAL, "invalid denominator range"};
-        }
+        validate(req);
         RPS_HIP(hipSetDevice(ctx->device));
         const rps::VariantInfo *var = rps::variant(ctx->variant);
         const int stride = 64 * var->words_per_lane;
+        const rps::LineDesc geometry = line_geometry(req);
+        const unsigned long long words = geometry.words;
         const long long A = req->numerator_bound;
-        const unsigned long long words = (2ULL * static_cast<unsigned long long>(A) + 64) / 64;
-        const unsigned long long valid_last =
-            2ULL * static_cast<unsigned long long>(A) - 64ULL * (words - 1) + 1;
-        const unsigned long long last_mask =
-            valid_last >= 64 ? ~0ULL : ((1ULL << valid_last) - 1);
         st.word_count = (2ULL * static_cast<unsigned long long>(A) + 32) / 32;
-
-        rps::SieveParams base{};
-        base.bound = A;
-        base.words = words;
-        base.last_mask = last_mask;
-        base.square = req->square_denominators ? 1 : 0;
         for (int i = 0; i < rps::kCandidates; ++i) {
             const long long p = ctx->primes[i];
-            base.primes[i] = static_cast<int>(p);
-            base.neg_bound_mod[i] = static_cast<int>(((-A) % p + p) % p);
+            ctx->h_negmod[i] = static_cast<int>(((-A) % p + p) % p);
         }
         const auto t_workspace = Clock::now();
         st.plan_ms = ms_between(t_plan, t_workspace);
         ensure_capacity(ctx);
-        base.capacity = ctx->capacity;
+        RPS_HIP(hipMemcpyAsync(ctx->d_negmod, ctx->h_negmod, rps::kCandidates * sizeof(int),
+                               hipMemcpyHostToDevice, ctx->compute));
         const auto t_basis = Clock::now();
         st.workspace_ms = ms_between(t_workspace, t_basis);
         prepare_curve(ctx, req, &st);
-        base.resid = ctx->d_resid;
         const auto t_setup = Clock::now();
         st.basis_wall_ms = ms_between(t_basis, t_setup);
+
+        rps::SieveParams base{};
+        base.line = geometry;
+        base.line.resid = ctx->d_resid;
+        base.line.neg_bound_mod = ctx->d_negmod;
+        base.primes = ctx->d_primes;
+        base.out.capacity = ctx->capacity;
 
         const long long last_t = req->den_last;
         long long next_t = req->den_first;
@@ -556,23 +648,17 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
             Slot &s = ctx->slots[si];
             ensure_cursors(s, bt.tiles * std::max(var->waves, rps::kDirectWaves));
             rps::SieveParams prm = base;
-            prm.first_t = bt.first_t;
-            prm.n_t = bt.n_t;
-            prm.tiles_per_t = bt.tiles_per_t;
-            prm.tile_words = bt.tile_words;
+            prm.line = bt.line;
             prm.resume = resume ? 1 : 0;
-            prm.out_num = s.d_num;
-            prm.out_den = s.d_den;
-            prm.state = s.d_state;
+            prm.out.num = s.d_num;
+            prm.out.den = s.d_den;
+            prm.out.req = nullptr;
+            prm.out.state = s.d_state;
             prm.cursors = s.d_cursors;
             RPS_HIP(rps::launch_reset(s.d_state, ctx->compute));
             RPS_HIP(hipEventRecord(s.k_start, ctx->compute));
-            if (bt.direct) {
-                prm.tile_t = bt.tile_t;
-                prm.magic_words = words == 1 ? 0xFFFFFFFFu
-                    : static_cast<unsigned>(0x100000000ULL / words);
-                RPS_HIP(rps::launch_direct(prm, static_cast<unsigned>(bt.tiles),
-                                           ctx->compute));
+            if (bt.line.direct) {
+                RPS_HIP(rps::launch_direct(prm, static_cast<unsigned>(bt.tiles), ctx->compute));
             } else {
                 RPS_HIP(var->launch(prm, static_cast<unsigned>(bt.tiles), ctx->compute));
             }
@@ -583,7 +669,7 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
             ++st.launches;
             if (!resume) {
                 ++st.batches;
-                ctx->last_sites += line_sites * bt.n_t;
+                ctx->last_sites += line_sites * bt.line.n_t;
             }
         };
         auto next_batch = [&](Batch &bt) -> bool {
@@ -591,18 +677,22 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
                 return false;
             }
             long long n = std::min<long long>(ctx->batch_size, last_t - next_t + 1);
+            bt.line = base.line;
             if (words <= ctx->direct_words) {
-                bt = make_direct_batch(static_cast<int>(next_t), static_cast<int>(n), words);
-                next_t += n;
-                return true;
+                direct_tiling(bt.line);
+            } else {
+                unsigned long long chunks = line_chunks(ctx->min_tiles, n, words, stride,
+                                                        var->waves);
+                if (chunks > kMaxTilesPerLaunch) {
+                    throw ApiError{RPS_EINVAL, "maximum height exceeds GPU limit"};
+                }
+                n = std::min<long long>(n, static_cast<long long>(kMaxTilesPerLaunch / chunks));
+                chunks = line_chunks(ctx->min_tiles, n, words, stride, var->waves);
+                row_tiling(bt.line, chunks, stride);
             }
-            unsigned long long chunks = line_chunks(ctx, n, words, stride, var->waves);
-            if (chunks > kMaxTilesPerLaunch) {
-                throw ApiError{RPS_EINVAL, "maximum height exceeds GPU limit"};
-            }
-            n = std::min<long long>(n, static_cast<long long>(kMaxTilesPerLaunch / chunks));
-            chunks = line_chunks(ctx, n, words, stride, var->waves);
-            bt = make_batch(static_cast<int>(next_t), static_cast<int>(n), chunks, words, stride);
+            bt.line.first_t = static_cast<int>(next_t);
+            bt.line.n_t = static_cast<int>(n);
+            bt.tiles = tile_count(bt.line);
             next_t += n;
             return true;
         };
@@ -667,8 +757,8 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
                 break;
             }
             if (done) {
-                const Batch &bt = inflight[cur];
-                if (done(user, bt.first_t, bt.first_t + bt.n_t - 1,
+                const rps::LineDesc &bl = inflight[cur].line;
+                if (done(user, bl.first_t, bl.first_t + bl.n_t - 1,
                          st.survivor_count - before) != 0) {
                     cancelled = true;
                     break;
@@ -690,16 +780,274 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
         ctx->last_launches = st.launches;
         return cancelled ? RPS_ECANCELLED : RPS_OK;
     } catch (const HipError &e) {
-        ctx->error = op_name(e.op) + ": " + hipGetErrorString(e.code);
-        (void)hipStreamSynchronize(ctx->compute);
-        (void)hipStreamSynchronize(ctx->copy);
-        ctx->curve_ready = false;
-        return RPS_EDEVICE;
+        return fail(ctx, e);
     } catch (const ApiError &e) {
-        ctx->error = e.message;
-        (void)hipStreamSynchronize(ctx->compute);
-        (void)hipStreamSynchronize(ctx->copy);
-        return e.code;
+        return fail(ctx, e);
+    } catch (const std::bad_alloc &) {
+        ctx->error = "out of host memory";
+        return RPS_ENOMEM;
+    }
+}
+
+int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
+                   rps_multi_sink sink, void *user, rps_stats *stats) {
+    if (!ctx) {
+        return RPS_EINVAL;
+    }
+    rps_stats local{};
+    rps_stats &st = stats ? *stats : local;
+    st = rps_stats{};
+    ctx->error.clear();
+    ctx->last_kernel_ms = 0.0;
+    ctx->last_launches = 0;
+    ctx->last_sites = 0.0;
+    const auto t_plan = Clock::now();
+    try {
+        if (!sink || (!requests && n_requests)) {
+            throw ApiError{RPS_EINVAL, "null requests or sink"};
+        }
+        if (n_requests > (size_t(1) << 20)) {
+            throw ApiError{RPS_EINVAL, "too many requests in one call (at most 2^20)"};
+        }
+        if (n_requests == 0) {
+            return RPS_OK;
+        }
+        const int n = static_cast<int>(n_requests);
+        int n_coef = 2;
+        for (int r = 0; r < n; ++r) {
+            validate(&requests[r]);
+            n_coef = std::max(n_coef, requests[r].n_coefficients);
+        }
+        RPS_HIP(hipSetDevice(ctx->device));
+        const rps::VariantInfo *var = rps::variant(ctx->variant);
+        const int stride = 64 * var->words_per_lane;
+
+        // Geometry and tiling per curve.  Long lines are cut into chunks of
+        // at most 2^18 words so one curve's tiles stay small next to the rest
+        // of the launch.
+        std::vector<rps::LineDesc> lines(n);
+        std::vector<double> cost(n);
+        double sites = 0.0;
+        for (int r = 0; r < n; ++r) {
+            rps::LineDesc L = line_geometry(&requests[r]);
+            L.first_t = requests[r].den_first;
+            L.n_t = requests[r].den_last - requests[r].den_first + 1;
+            if (L.words <= ctx->direct_words) {
+                direct_tiling(L);
+            } else {
+                const unsigned long long chunks = std::max(
+                    (L.words + (1ULL << 18) - 1) >> 18, 1ULL);
+                row_tiling(L, chunks, stride);
+            }
+            lines[r] = L;
+            const double line_sites = 2.0 * static_cast<double>(L.bound) + 1.0;
+            sites += line_sites * L.n_t;
+            cost[r] = (line_sites + 150000.0) * L.n_t;  // + per-line setup
+        }
+        std::vector<int> order(n);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return cost[a] > cost[b]; });
+
+        // Per-curve tables, one batched build for all curves.
+        ctx->m_coef.ensure(static_cast<size_t>(n) * rps::kCandidates * n_coef);
+        ctx->m_chi.ensure(static_cast<size_t>(n) * rps::kCandidates * rps::kChiWords);
+        ctx->m_resid.ensure(static_cast<size_t>(n) * rps::kResidTableWords);
+        ctx->m_negmod.ensure(static_cast<size_t>(n) * rps::kCandidates);
+        ctx->m_curves.ensure(n);
+        ctx->m_tile_start.ensure(n + 1);
+        ctx->m_failed.ensure(n);
+        ctx->m_head.ensure(1);
+        ctx->m_req.ensure(ctx->capacity);
+        ensure_capacity(ctx);
+        std::vector<int> coef(static_cast<size_t>(n) * rps::kCandidates * n_coef);
+        std::vector<int> negmod(static_cast<size_t>(n) * rps::kCandidates);
+        std::vector<rps::CurveDesc> curves(n);
+        std::vector<unsigned long long> tile_start(n + 1, 0);
+        for (int k = 0; k < n; ++k) {
+            const int r = order[k];
+            coefficient_residues(&requests[r], ctx->primes, n_coef,
+                                 &coef[static_cast<size_t>(k) * rps::kCandidates * n_coef]);
+            for (int i = 0; i < rps::kCandidates; ++i) {
+                const long long p = ctx->primes[i];
+                negmod[static_cast<size_t>(k) * rps::kCandidates + i] =
+                    static_cast<int>(((-requests[r].numerator_bound) % p + p) % p);
+            }
+            rps::CurveDesc &C = curves[k];
+            C.line = lines[r];
+            C.line.resid = ctx->m_resid.ptr + static_cast<size_t>(k) * rps::kResidTableWords;
+            C.line.neg_bound_mod = ctx->m_negmod.ptr + static_cast<size_t>(k) * rps::kCandidates;
+            C.request = r;
+            C.tile_start = tile_start[k];
+            tile_start[k + 1] = tile_start[k] + tile_count(C.line);
+        }
+        const unsigned long long total_tiles = tile_start[n];
+        const auto t_workspace = Clock::now();
+        st.plan_ms = ms_between(t_plan, t_workspace);
+        RPS_HIP(hipMemcpy(ctx->m_coef.ptr, coef.data(), coef.size() * sizeof(int),
+                          hipMemcpyHostToDevice));
+        RPS_HIP(hipMemcpy(ctx->m_negmod.ptr, negmod.data(), negmod.size() * sizeof(int),
+                          hipMemcpyHostToDevice));
+        RPS_HIP(hipMemcpy(ctx->m_curves.ptr, curves.data(), n * sizeof(rps::CurveDesc),
+                          hipMemcpyHostToDevice));
+        RPS_HIP(hipMemcpy(ctx->m_tile_start.ptr, tile_start.data(),
+                          (n + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice));
+        RPS_HIP(hipMemsetAsync(ctx->m_failed.ptr, 0, n * sizeof(int), ctx->compute));
+        RPS_HIP(hipMemsetAsync(ctx->m_head.ptr, 0, sizeof(unsigned long long), ctx->compute));
+        const auto t_basis = Clock::now();
+        st.workspace_ms = ms_between(t_workspace, t_basis);
+        RPS_HIP(hipEventRecord(ctx->basis_start, ctx->compute));
+        RPS_HIP(rps::launch_chi(ctx->m_coef.ptr, n_coef, n, ctx->d_primes, ctx->m_chi.ptr,
+                                ctx->compute));
+        RPS_HIP(rps::launch_resid(ctx->d_primes, ctx->m_chi.ptr, n, ctx->m_resid.ptr,
+                                  ctx->compute));
+        RPS_HIP(hipEventRecord(ctx->basis_end, ctx->compute));
+        RPS_HIP(hipEventSynchronize(ctx->basis_end));
+        float bms = 0.0f;
+        RPS_HIP(hipEventElapsedTime(&bms, ctx->basis_start, ctx->basis_end));
+        st.basis_ms = bms;
+        const auto t_sieve = Clock::now();
+        st.basis_wall_ms = ms_between(t_basis, t_sieve);
+
+        // Persistent launches until every tile was handed out.  Survivors
+        // are kept per request on the host until the end, because a request
+        // whose reservation fails is searched again on the plain path.
+        Slot &s = ctx->slots[0];
+        rps::MultiParams M{};
+        M.curves = ctx->m_curves.ptr;
+        M.tile_start = ctx->m_tile_start.ptr;
+        M.curve_failed = ctx->m_failed.ptr;
+        M.head = ctx->m_head.ptr;
+        M.total_tiles = total_tiles;
+        M.out.num = s.d_num;
+        M.out.den = s.d_den;
+        M.out.req = ctx->m_req.ptr;
+        M.out.state = s.d_state;
+        M.out.capacity = ctx->capacity;
+        M.primes = ctx->d_primes;
+        M.n_curves = n;
+        const unsigned grid = static_cast<unsigned>(ctx->cu_count) * 2u;
+        std::vector<std::vector<long long>> kept_num(n);
+        std::vector<std::vector<int>> kept_den(n);
+        unsigned long long head = 0;
+        while (head < total_tiles) {
+            RPS_HIP(rps::launch_reset(s.d_state, ctx->compute));
+            RPS_HIP(hipEventRecord(s.k_start, ctx->compute));
+            RPS_HIP(var->launch_multi(M, grid, ctx->compute));
+            RPS_HIP(hipEventRecord(s.k_end, ctx->compute));
+            RPS_HIP(hipMemcpyAsync(s.h_state, s.d_state, sizeof(rps::DeviceState),
+                                   hipMemcpyDeviceToHost, ctx->compute));
+            RPS_HIP(hipMemcpyAsync(&head, ctx->m_head.ptr, sizeof(head),
+                                   hipMemcpyDeviceToHost, ctx->compute));
+            RPS_HIP(hipStreamSynchronize(ctx->compute));
+            ++st.launches;
+            float kms = 0.0f;
+            RPS_HIP(hipEventElapsedTime(&kms, s.k_start, s.k_end));
+            st.sieve_ms += kms;
+            const rps::DeviceState state = *s.h_state;
+            const unsigned long long valid = std::min(state.count, state.first_fail);
+            if (valid > ctx->capacity) {
+                throw ApiError{RPS_EINTERNAL, "streaming survivor count exceeds capacity"};
+            }
+            for (unsigned long long at = 0; at < valid;) {
+                const size_t k = static_cast<size_t>(std::min<unsigned long long>(
+                    valid - at, ctx->stage_capacity));
+                RPS_HIP(hipMemcpyAsync(ctx->h_num, s.d_num + at, k * sizeof(long long),
+                                       hipMemcpyDeviceToHost, ctx->copy));
+                RPS_HIP(hipMemcpyAsync(ctx->h_den, s.d_den + at, k * sizeof(int),
+                                       hipMemcpyDeviceToHost, ctx->copy));
+                RPS_HIP(hipMemcpyAsync(ctx->h_req, ctx->m_req.ptr + at, k * sizeof(int),
+                                       hipMemcpyDeviceToHost, ctx->copy));
+                RPS_HIP(hipStreamSynchronize(ctx->copy));
+                for (size_t i = 0; i < k; ++i) {
+                    const int r = ctx->h_req[i];
+                    if (r < 0 || r >= n) {
+                        throw ApiError{RPS_EINTERNAL, "survivor with an invalid request index"};
+                    }
+                    kept_num[r].push_back(ctx->h_num[i]);
+                    kept_den[r].push_back(ctx->h_den[i]);
+                }
+                at += k;
+            }
+            if (head > total_tiles) {
+                head = total_tiles;
+            }
+        }
+        std::vector<int> failed(n);
+        RPS_HIP(hipMemcpy(failed.data(), ctx->m_failed.ptr, n * sizeof(int),
+                          hipMemcpyDeviceToHost));
+        const double multi_kernel_ms = st.sieve_ms;
+        const uint64_t multi_launches = st.launches;
+
+        // Deliver in request order; a request that failed is searched again
+        // alone on the plain path, whose per-wave cursors resume exactly.
+        std::vector<char> is_failed(n, 0);
+        for (int k = 0; k < n; ++k) {
+            if (failed[k]) {
+                is_failed[curves[k].request] = 1;
+            }
+        }
+        bool cancelled = false;
+        for (int r = 0; r < n && !cancelled; ++r) {
+            if (is_failed[r]) {
+                struct Fwd {
+                    rps_multi_sink sink;
+                    void *user;
+                    int request;
+                    std::vector<int> req;
+                    double ms;
+                } fwd{sink, user, r, {}, 0.0};
+                auto forward = [](void *u, const int64_t *num, const int32_t *den,
+                                  size_t count) -> int {
+                    Fwd &f = *static_cast<Fwd *>(u);
+                    f.req.assign(count, f.request);
+                    const auto v0 = Clock::now();
+                    const int rc = f.sink(f.user, f.req.data(), num, den, count);
+                    f.ms += ms_between(v0, Clock::now());
+                    return rc;
+                };
+                rps_stats one{};
+                const int rc = rps_sieve(ctx, &requests[r], forward, &fwd, &one);
+                st.survivor_count += one.survivor_count;
+                st.launches += one.launches;
+                st.sieve_ms += one.sieve_ms;
+                st.verification_ms += fwd.ms;
+                ++st.requeued;
+                if (rc == RPS_ECANCELLED) {
+                    cancelled = true;
+                } else if (rc != RPS_OK) {
+                    return rc;
+                }
+                continue;
+            }
+            const size_t count = kept_num[r].size();
+            for (size_t at = 0; at < count && !cancelled;) {
+                const size_t k = std::min(count - at, ctx->stage_capacity);
+                for (size_t i = 0; i < k; ++i) {
+                    ctx->h_req[i] = r;
+                }
+                const auto v0 = Clock::now();
+                const int rc = sink(user, ctx->h_req,
+                                    reinterpret_cast<const int64_t *>(kept_num[r].data() + at),
+                                    reinterpret_cast<const int32_t *>(kept_den[r].data() + at), k);
+                st.verification_ms += ms_between(v0, Clock::now());
+                st.survivor_count += k;
+                at += k;
+                if (rc != 0) {
+                    cancelled = true;
+                }
+            }
+        }
+        st.batches = static_cast<uint64_t>(n);
+        st.sieve_wall_ms = ms_between(t_sieve, Clock::now()) - st.verification_ms;
+        ctx->last_kernel_ms = multi_kernel_ms;
+        ctx->last_launches = multi_launches;
+        ctx->last_sites = sites;
+        return cancelled ? RPS_ECANCELLED : RPS_OK;
+    } catch (const HipError &e) {
+        return fail(ctx, e);
+    } catch (const ApiError &e) {
+        return fail(ctx, e);
     } catch (const std::bad_alloc &) {
         ctx->error = "out of host memory";
         return RPS_ENOMEM;

@@ -1,23 +1,26 @@
 // gfx950 kernels of the modular square sieve.
 //
-//   chi_kernel     chi_p bitmaps of one curve (src/sieve_plan.cpp:115-138 is
-//                  the host routine it replaces)
+//   chi_kernel     chi_p bitmaps of a batch of curves (src/sieve_plan.cpp:
+//                  115-138 is the host routine it replaces)
 //   resid_kernel   per (p, b mod p) residue strings; replaces the reference's
 //                  p x p 32-bit basis table (basis_kernel, src/sieve.cu:141-172)
-//                  with a 72-byte string per (p, b mod p) that the sieve
-//                  kernel expands into rows in LDS
+//                  with a 72-byte string per (p, b mod p)
 //   sieve_kernel   the hot loop (src/sieve.cu:221-388 is the kernel it
-//                  replaces): one workgroup per (denominator, word chunk)
+//                  replaces): one workgroup per (denominator, word chunk) row
 //                  tile; dense primes ANDed from LDS rows with wave-uniform
 //                  offsets; remaining primes applied to surviving words only;
 //                  survivors compacted with a wave prefix sum and one atomic
-//                  reservation per wave, with a resumable per-wave cursor.
+//                  reservation per wave, with a resumable per-wave cursor
+//   direct_kernel  short lines: tiles of whole lines, windows evaluated
+//                  directly from the residue strings
+//   multi_kernel   many curves in one persistent launch: workgroups pull row
+//                  and short-line tiles of all curves from a device counter
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 
-#include "sieve_layout.hpp"
 #include "sieve_kernels.hpp"
+#include "sieve_layout.hpp"
 
 namespace rps {
 namespace {
@@ -44,29 +47,431 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
     const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(v), lane);
     const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(v >> 32), lane);
     return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
+__device__ __forceinline__ int denominator(const LineDesc &L, long long t) {
+    return static_cast<int>(L.square ? t * t : t);
+}
+
+// Wave-level survivor emission.  Lane l holds U masks; mask u covers the 64
+// numerators of word first_word + 64 u on denominator b.  A prefix sum of
+// per-lane counts gives every lane its slots under one atomic reservation per
+// wave, all or nothing: when the reservation does not fit, first_fail records
+// its start, nothing is written and false is returned, so the wave can stop
+// and resume at the same iteration after the host drains the buffer.
+template <int U>
+__device__ __forceinline__ bool emit_survivors(const OutDesc &O, long long bound,
+                                               const uint64_t (&m)[U],
+                                               uint64_t first_word, int b,
+                                               int request, int lane) {
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        cnt += __popcll(m[u]);
+    }
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t v = __shfl_up(incl, d, 64);
+        if (lane >= d) {
+            incl += v;
+        }
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    unsigned long long start = 0;
+    if (lane == 0) {
+        start = atomicAdd(&O.state->count, static_cast<unsigned long long>(total));
+    }
+    start = __shfl(start, 0, 64);
+    if (start + total > O.capacity) {
+        if (lane == 0) {
+            atomicMin(&O.state->first_fail, start);
+            atomicOr(&O.state->full, 1u);
+        }
+        return false;
+    }
+    unsigned long long pos = start + incl - cnt;
+    const uint64_t neg_a = static_cast<uint64_t>(-bound);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        uint64_t mm = m[u];
+        const uint64_t a0 = neg_a + 64 * (first_word + 64 * u);
+        while (mm) {
+            const int j = __ffsll(static_cast<long long>(mm)) - 1;
+            mm &= mm - 1;
+            O.num[pos] = static_cast<long long>(a0 + j);
+            O.den[pos] = b;
+            if (O.req) {
+                O.req[pos] = request;
+            }
+            ++pos;
+        }
+    }
+    return true;
+}
+
 }  // namespace
 
-// One workgroup of 512 threads per candidate prime.  Thread x evaluates
-// f(x) mod p by Horner's rule on the coefficient residues and tests it
-// against the table of squares mod p (zero counts as a square).
+// ---------------------------------------------------------------------------
+// Row tile: one denominator, one chunk [ts, te) of its line.
+//   U      64-bit words per lane per iteration (a wave covers 64 U words)
+//   FAST   selected primes ANDed densely for every word; the other
+//          32 - FAST are applied only to words with bits left
+//   WAVES  waves per workgroup; each wave sweeps a contiguous word range
+// ---------------------------------------------------------------------------
+template <int U, int FAST, int WAVES>
+struct RowTile {
+    static constexpr int kThreads = WAVES * 64;
+    static constexpr int kStride = 64 * U;
+    static constexpr int kSlot = kMaxPrime + 64 * U;  // >= p + 64 U - 1 words
+    static constexpr int kLate = kSelected - FAST;
+    // LDS layout, in 64-bit words: rows, residue strings, then six arrays of
+    // 32 uint32 (p, (-A) mod p, b mod p, candidate index, ts mod p, magic).
+    static constexpr int kCext = FAST * kSlot;
+    static constexpr int kSmall = kCext + kSelected * 10;
+    static constexpr int kLdsWords = kSmall + 3 * kSelected;
+    static_assert(kStride < 293, "a row step must stay below every prime");
+    static_assert(FAST >= 1 && FAST <= kSelected, "dense prime count");
+    static_assert(kLate <= 32, "late primes must fit a half wave");
+
+    // Word range of wave `wave` in [ts, te), in whole strides.
+    __device__ static void wave_range(uint64_t ts, uint64_t te, int wave,
+                                      uint64_t &ws, uint64_t &we) {
+        const uint64_t per =
+            ((te - ts + static_cast<uint64_t>(kStride) * WAVES - 1)
+             / (static_cast<uint64_t>(kStride) * WAVES)) * kStride;
+        ws = ts + per * wave;
+        we = ws + per < te ? ws + per : te;
+        if (ws > we) {
+            ws = we;
+        }
+    }
+
+    // Prime selection, residue strings and dense rows for denominator b;
+    // every thread of the workgroup calls it.
+    __device__ static void setup(uint64_t *pool, int b, uint64_t ts,
+                                 const LineDesc &L, const int *primes,
+                                 int tid, int lane, int wave) {
+        uint32_t *s_p = reinterpret_cast<uint32_t *>(pool + kSmall);
+        uint32_t *s_c = s_p + kSelected;
+        uint32_t *s_u = s_c + kSelected;
+        uint32_t *s_idx = s_u + kSelected;
+        uint32_t *s_base = s_idx + kSelected;
+        uint32_t *s_magic = s_base + kSelected;
+        uint64_t *cext = pool + kCext;
+        // src/sieve_plan.cpp:75-86: the 32 largest candidates not dividing b.
+        // Lane i tests candidate i; its slot is the number of qualifying
+        // candidates above it.
+        if (wave == 0) {
+            const bool cand = lane < kCandidates;
+            const uint32_t p = cand ? static_cast<uint32_t>(primes[lane]) : 1u;
+            const uint32_t u = cand ? static_cast<uint32_t>(b) % p : 0u;
+            const bool ok = cand && u != 0;
+            const unsigned long long mask = __ballot(ok);
+            if (ok) {
+                const int rank = __popcll(mask >> (lane + 1));
+                if (rank < kSelected) {
+                    s_p[rank] = p;
+                    s_c[rank] = static_cast<uint32_t>(L.neg_bound_mod[lane]);
+                    s_u[rank] = u;
+                    s_idx[rank] = lane;
+                    s_magic[rank] = static_cast<uint32_t>(0x100000000ULL / p);
+                    s_base[rank] = static_cast<uint32_t>(ts % p);
+                }
+            }
+        }
+        __syncthreads();
+        for (int k = tid; k < kSelected * kResidWords; k += kThreads) {
+            const int s = k / kResidWords;
+            const int w = k - s * kResidWords;
+            cext[s * 10 + w] = L.resid[(static_cast<size_t>(s_idx[s]) * kMaxPrime
+                                        + s_u[s]) * kResidWords + w];
+        }
+        __syncthreads();
+        // Dense rows: entry e of row s is word e of the line mod p, the window
+        // of the residue string at (-A + 64 e) mod p; entries e >= p repeat
+        // the row so that reads never wrap.
+        for (int s = 0; s < FAST; ++s) {
+            const uint32_t p = s_p[s];
+            const uint32_t c = s_c[s];
+            const uint32_t magic = s_magic[s];
+            const uint64_t *cx = cext + s * 10;
+            const int n = static_cast<int>(p) + kStride - 1;
+            for (int e = tid; e < n; e += kThreads) {
+                const uint32_t o = mod_small(c + 64u * static_cast<uint32_t>(e), p, magic);
+                pool[s * kSlot + e] = window64(cx, o);
+            }
+        }
+        __syncthreads();
+    }
+
+    // One wave sweeps [ws, we).  Returns we when done, or the start of the
+    // iteration whose survivors did not fit the buffer.
+    __device__ static uint64_t sweep(const uint64_t *pool, int b, uint64_t ts,
+                                     uint64_t ws, uint64_t we, const LineDesc &L,
+                                     const OutDesc &O, int request, int lane) {
+        const uint32_t *s_p = reinterpret_cast<const uint32_t *>(pool + kSmall);
+        const uint32_t *s_c = s_p + kSelected;
+        const uint32_t *s_base = s_c + 3 * kSelected;
+        const uint32_t *s_magic = s_base + kSelected;
+        const uint64_t *cext = pool + kCext;
+        // Wave-uniform row offsets: word ws mod p for each dense prime.
+        uint32_t off[FAST];
+        uint32_t prime[FAST];
+        {
+            const uint32_t rel = static_cast<uint32_t>(ws - ts);
+            uint32_t mine = 0;
+            if (lane < FAST) {
+                const uint32_t p = s_p[lane];
+                const uint32_t v = mod_small(rel, p, s_magic[lane]) + s_base[lane];
+                mine = v >= p ? v - p : v;
+            }
+#pragma unroll
+            for (int s = 0; s < FAST; ++s) {
+                off[s] = __builtin_amdgcn_readlane(mine, s);
+                prime[s] = uniform(s_p[s]);
+            }
+        }
+        // Late-prime constants: lane j of each 32-lane half owns prime
+        // FAST + j.
+        const int lj = lane & 31;
+        const bool late_lane = lj < kLate;
+        uint32_t lp = 1, lmagic = 0, lbase = 0, lc = 0;
+        const uint64_t *lcx = cext;
+        if (late_lane) {
+            lp = s_p[FAST + lj];
+            lmagic = s_magic[FAST + lj];
+            lbase = s_base[FAST + lj];
+            lc = s_c[FAST + lj];
+            lcx = cext + (FAST + lj) * 10;
+        }
+
+        const uint64_t last_word = L.words - 1;
+        uint64_t W = ws;
+        for (; W < we; W += kStride) {
+            uint64_t m[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                m[u] = ~0ULL;
+            }
+#pragma unroll
+            for (int s = 0; s < FAST; ++s) {
+                const uint64_t *r = pool + s * kSlot + off[s] + lane;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    m[u] &= r[64 * u];
+                }
+                const uint32_t next = off[s] + kStride;
+                off[s] = next >= prime[s] ? next - prime[s] : next;
+            }
+            if (W + kStride >= we) {
+                // Last iteration of this wave: drop words past the wave's
+                // range and the sites past +A in the line's final word.
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint64_t w = W + lane + 64 * u;
+                    if (w >= we) {
+                        m[u] = 0;
+                    } else if (w == last_word) {
+                        m[u] &= L.last_mask;
+                    }
+                }
+            }
+            uint64_t any = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                any |= m[u];
+            }
+            if (!__ballot(any != 0)) {
+                continue;
+            }
+            // Remaining primes, only for words that still have bits set.  Up
+            // to two such words are taken per pass as wave-uniform values; in
+            // each 32-lane half, lane j evaluates late prime FAST + j for its
+            // word and the half ANDs the row words together.
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                uint64_t pend = __ballot(m[u] != 0);
+                while (pend) {
+                    const int l0 = __builtin_ctzll(pend);
+                    pend &= pend - 1;
+                    int l1 = -1;
+                    if (pend) {
+                        l1 = __builtin_ctzll(pend);
+                        pend &= pend - 1;
+                    }
+                    const int src = lane < 32 ? l0 : l1;
+                    uint64_t e = ~0ULL;
+                    if (late_lane && src >= 0) {
+                        const uint32_t rel =
+                            static_cast<uint32_t>(W - ts) + src + 64u * u;
+                        uint32_t wm = mod_small(rel, lp, lmagic) + lbase;
+                        wm = wm >= lp ? wm - lp : wm;
+                        e = window64(lcx, mod_small(lc + 64u * wm, lp, lmagic));
+                    }
+#pragma unroll
+                    for (int d = 1; d < 32; d <<= 1) {
+                        e &= __shfl_xor(e, d, 64);
+                    }
+                    const uint64_t r0 = readlane64(e, 0);
+                    const uint64_t r1 = readlane64(e, 32);
+                    if (lane == l0) {
+                        m[u] &= r0;
+                    }
+                    if (lane == l1) {
+                        m[u] &= r1;
+                    }
+                }
+            }
+            uint64_t live = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                live |= m[u];
+            }
+            if (!__ballot(live != 0)) {
+                continue;
+            }
+            if (!emit_survivors<U>(O, L.bound, m, W + lane, b, request, lane)) {
+                return W;
+            }
+        }
+        return we;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Short-line tile: up to NB whole lines of one curve, denominators t0 .. t0 +
+// nb - 1.  No row expansion: each lane takes one (denominator, word) pair and
+// ANDs the window of every selected prime directly, one Barrett reduction and
+// one funnel shift per prime, with a wave-uniform early exit once every
+// lane's word is empty.
+// ---------------------------------------------------------------------------
+template <int WAVES, int NB>
+struct DirectTile {
+    // LDS: residue strings [NB][32][10] words, then {p | c << 16, magic}
+    // pairs [NB][32] as uint32.
+    static constexpr int kPrm = NB * kSelected * 10;
+    static constexpr int kLdsWords = kPrm + NB * kSelected;
+
+    __device__ static void wave_range(uint32_t total, int wave, uint32_t &qs,
+                                      uint32_t &qe) {
+        const uint32_t per = ((total + 64u * WAVES - 1) / (64u * WAVES)) * 64u;
+        qs = per * wave;
+        qe = qs + per < total ? qs + per : total;
+        if (qs > qe) {
+            qs = qe;
+        }
+    }
+
+    // Prime selection per denominator (src/sieve_plan.cpp:75-86), one wave
+    // per denominator; the lane that ranks a prime stages its string.
+    __device__ static void setup(uint64_t *pool, int t0, int nb,
+                                 const LineDesc &L, const int *primes,
+                                 int lane, int wave) {
+        uint32_t *prm = reinterpret_cast<uint32_t *>(pool + kPrm);
+        for (int bl = wave; bl < nb; bl += WAVES) {
+            const uint32_t b = static_cast<uint32_t>(
+                denominator(L, static_cast<long long>(t0) + bl));
+            const bool cand = lane < kCandidates;
+            const uint32_t p = cand ? static_cast<uint32_t>(primes[lane]) : 1u;
+            const uint32_t u = cand ? b % p : 0u;
+            const bool ok = cand && u != 0;
+            const unsigned long long mask = __ballot(ok);
+            if (ok) {
+                const int rank = __popcll(mask >> (lane + 1));
+                if (rank < kSelected) {
+                    const int slot = bl * kSelected + rank;
+                    prm[2 * slot] = p | (static_cast<uint32_t>(L.neg_bound_mod[lane]) << 16);
+                    prm[2 * slot + 1] = static_cast<uint32_t>(0x100000000ULL / p);
+                    const unsigned long long *src =
+                        L.resid + (static_cast<size_t>(lane) * kMaxPrime + u) * kResidWords;
+#pragma unroll
+                    for (int w = 0; w < kResidWords; ++w) {
+                        pool[slot * 10 + w] = src[w];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // One wave sweeps pairs [qs, qe) (pair q = line q / words, word q mod
+    // words).  Returns qe when done, or the iteration that did not fit.
+    __device__ static uint32_t sweep(const uint64_t *pool, int t0, uint32_t qs,
+                                     uint32_t qe, const LineDesc &L,
+                                     const OutDesc &O, int request, int lane) {
+        const uint32_t *prm = reinterpret_cast<const uint32_t *>(pool + kPrm);
+        const uint32_t words = static_cast<uint32_t>(L.words);
+        uint32_t Q = qs;
+        for (; Q < qe; Q += 64) {
+            const uint32_t q = Q + lane;
+            const bool valid = q < qe;
+            uint32_t bl = __umulhi(q, L.magic_words);
+            uint32_t w = q - bl * words;
+            if (w >= words) {
+                w -= words;
+                ++bl;
+            }
+            if (!valid) {
+                bl = 0;
+                w = 0;
+            }
+            uint64_t m[1];
+            m[0] = !valid ? 0ULL : (w == words - 1 ? L.last_mask : ~0ULL);
+            const uint32_t *pp = prm + bl * kSelected * 2;
+            const uint64_t *cx = pool + bl * kSelected * 10;
+            for (int s = 0; s < kSelected; ++s) {
+                if (s >= 8 && (s & 3) == 0 && !__ballot(m[0] != 0)) {
+                    break;
+                }
+                const uint32_t pc = pp[2 * s];
+                const uint32_t p = pc & 0xffffu;
+                const uint32_t o = mod_small((pc >> 16) + 64u * w, p, pp[2 * s + 1]);
+                m[0] &= window64(cx + s * 10, o);
+            }
+            if (!__ballot(m[0] != 0)) {
+                continue;
+            }
+            const int b = denominator(L, static_cast<long long>(t0) + bl);
+            if (!emit_survivors<1>(O, L.bound, m, w, b, request, lane)) {
+                return Q;
+            }
+        }
+        return qe;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Per-curve tables
+// ---------------------------------------------------------------------------
+
+// One workgroup of 512 threads per (candidate prime, curve).  Thread x
+// evaluates f(x) mod p by Horner's rule on the coefficient residues (padded
+// with zeros above the degree) and tests it against the squares mod p (zero
+// counts as a square).
 __global__ __launch_bounds__(512) void chi_kernel(
-    const int *__restrict__ coef_res,  // [36][n_coef]
+    const int *__restrict__ coef_res,  // [curves][36][n_coef]
     int n_coef, const int *__restrict__ primes,
-    unsigned long long *__restrict__ chi) {  // [36][8]
+    unsigned long long *__restrict__ chi) {  // [curves][36][8]
     __shared__ unsigned char square[kMaxPrime];
     __shared__ int res[kMaxCoefficients];
     const int i = blockIdx.x;
+    const size_t curve = blockIdx.y;
     const int x = threadIdx.x;
     const int p = primes[i];
     square[x] = 0;
     for (int k = x; k < n_coef; k += blockDim.x) {
-        res[k] = coef_res[i * n_coef + k];
+        res[k] = coef_res[(curve * kCandidates + i) * n_coef + k];
     }
     __syncthreads();
     if (x < p) {
@@ -83,23 +488,25 @@ __global__ __launch_bounds__(512) void chi_kernel(
     }
     unsigned long long word = __ballot(bit);
     if ((x & 63) == 0) {
-        chi[i * kChiWords + (x >> 6)] = word;
+        chi[(curve * kCandidates + i) * kChiWords + (x >> 6)] = word;
     }
 }
 
-// Residue strings: one wave per (prime index i, u).  For u = 0 (p | b) the
-// string is never read and is stored as zero.  Bit z of the string is
+// Residue strings: one wave per (prime index i, u, curve).  For u = 0 (p | b)
+// the string is never read and is stored as zero.  Bit z of the string is
 // chi_p((z mod p) * u^-1 mod p); ballot packs 64 consecutive z per word.
 __global__ __launch_bounds__(256) void resid_kernel(
     const int *__restrict__ primes,
     const unsigned long long *__restrict__ chi,
-    unsigned long long *__restrict__ resid) {  // [36][512][9]
+    unsigned long long *__restrict__ resid) {  // [curves][36][512][9]
     const int i = blockIdx.x;
+    const size_t curve = blockIdx.z;
     const int lane = threadIdx.x & 63;
     const int u = blockIdx.y * 4 + (threadIdx.x >> 6);
     const uint32_t p = primes[i];
-    unsigned long long *dst =
-        resid + (static_cast<size_t>(i) * kMaxPrime + u) * kResidWords;
+    const unsigned long long *ch = chi + (curve * kCandidates + i) * kChiWords;
+    unsigned long long *dst = resid + curve * kResidTableWords
+        + (static_cast<size_t>(i) * kMaxPrime + u) * kResidWords;
     if (u == 0 || u >= static_cast<int>(p)) {
         if (lane < kResidWords) {
             dst[lane] = 0;
@@ -121,9 +528,9 @@ __global__ __launch_bounds__(256) void resid_kernel(
         if (z >= p) {
             z -= p;  // z < 576 < 2p
         }
-        uint32_t y = z * inv % p;
-        int bit = static_cast<int>((chi[i * kChiWords + (y >> 6)] >> (y & 63)) & 1);
-        unsigned long long word = __ballot(bit);
+        const uint32_t y = z * inv % p;
+        const int bit = static_cast<int>((ch[y >> 6] >> (y & 63)) & 1);
+        const unsigned long long word = __ballot(bit);
         if (lane == k) {
             mine = word;
         }
@@ -140,110 +547,31 @@ __global__ void reset_state_kernel(DeviceState *state) {
     state->pad = 0;
 }
 
-// Wave-level survivor emission.  Lane l holds U masks; mask u covers the 64
-// numerators of word first_word + 64 u on denominator b.  A prefix sum of
-// per-lane counts gives every lane its slots under one atomic reservation per
-// wave, all or nothing: when the reservation does not fit, first_fail records
-// its start, nothing is written and false is returned so the wave can stop
-// and resume at the same iteration after the host drains the buffer.
-template <int U>
-__device__ __forceinline__ bool emit_survivors(const SieveParams &P,
-                                               const uint64_t (&m)[U],
-                                               uint64_t first_word, int b,
-                                               int lane) {
-    uint32_t cnt = 0;
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        cnt += __popcll(m[u]);
-    }
-    uint32_t incl = cnt;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t v = __shfl_up(incl, d, 64);
-        if (lane >= d) {
-            incl += v;
-        }
-    }
-    const uint32_t total = __shfl(incl, 63, 64);
-    unsigned long long start = 0;
-    if (lane == 0) {
-        start = atomicAdd(&P.state->count, static_cast<unsigned long long>(total));
-    }
-    start = __shfl(start, 0, 64);
-    if (start + total > P.capacity) {
-        if (lane == 0) {
-            atomicMin(&P.state->first_fail, start);
-            atomicOr(&P.state->full, 1u);
-        }
-        return false;
-    }
-    unsigned long long pos = start + incl - cnt;
-    const uint64_t neg_a = static_cast<uint64_t>(-P.bound);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        uint64_t mm = m[u];
-        const uint64_t a0 = neg_a + 64 * (first_word + 64 * u);
-        while (mm) {
-            const int j = __ffsll(static_cast<long long>(mm)) - 1;
-            mm &= mm - 1;
-            P.out_num[pos] = static_cast<long long>(a0 + j);
-            P.out_den[pos] = b;
-            ++pos;
-        }
-    }
-    return true;
-}
+// ---------------------------------------------------------------------------
+// Plain launches (one curve, one denominator batch)
+// ---------------------------------------------------------------------------
 
-// The sieve.  Template parameters:
-//   U      64-bit words per lane per iteration (a wave covers 64 U words)
-//   FAST   selected primes ANDed densely for every word; the other
-//          32 - FAST are applied only to words with bits left
-//   WAVES  waves per workgroup; each wave sweeps a contiguous word range
 template <int U, int FAST, int WAVES>
-__global__ __launch_bounds__(WAVES * 64) void sieve_kernel(
-    const SieveParams P) {
-    constexpr int kThreads = WAVES * 64;
-    constexpr int kSlot = kMaxPrime + 64 * U;  // >= p + 64 U - 1 words
-    constexpr int kStride = 64 * U;
-    static_assert(kStride < 293, "a row step must stay below every prime");
-    static_assert(FAST >= 1 && FAST <= kSelected, "dense prime count");
-
-    __shared__ uint64_t rows[FAST * kSlot];
-    __shared__ uint64_t cext[kSelected * 10];
-    __shared__ uint32_t s_p[kSelected];
-    __shared__ uint32_t s_c[kSelected];
-    __shared__ uint32_t s_u[kSelected];
-    __shared__ uint32_t s_idx[kSelected];
-    __shared__ uint32_t s_base[kSelected];
-    __shared__ uint32_t s_magic[kSelected];
+__global__ __launch_bounds__(WAVES * 64) void sieve_kernel(const SieveParams P) {
+    using Tile = RowTile<U, FAST, WAVES>;
+    __shared__ uint64_t pool[Tile::kLdsWords];
     __shared__ int s_live;
-
+    const LineDesc &L = P.line;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = static_cast<int>(uniform(tid >> 6));
     const int tile = blockIdx.x;
-    const int t_index = tile / P.tiles_per_t;
-    const int chunk = tile - t_index * P.tiles_per_t;
-    const long long t = static_cast<long long>(P.first_t) + t_index;
-    const int b = static_cast<int>(P.square ? t * t : t);
-
-    // Word range of this tile and of this wave (multiples of the stride).
-    const uint64_t ts = static_cast<uint64_t>(chunk) * P.tile_words;
-    const uint64_t te = ts + P.tile_words < P.words ? ts + P.tile_words : P.words;
-    const uint64_t per =
-        ((te - ts + static_cast<uint64_t>(kStride) * WAVES - 1)
-         / (static_cast<uint64_t>(kStride) * WAVES)) * kStride;
-    uint64_t ws = ts + per * wave;
-    const uint64_t we = ws + per < te ? ws + per : te;
-    if (ws > we) {
-        ws = we;
-    }
-    unsigned long long *cursor =
-        P.cursors + static_cast<size_t>(tile) * WAVES + wave;
+    const int t_index = tile / L.tiles_per_t;
+    const int chunk = tile - t_index * L.tiles_per_t;
+    const int b = denominator(L, static_cast<long long>(L.first_t) + t_index);
+    const uint64_t ts = static_cast<uint64_t>(chunk) * L.tile_words;
+    const uint64_t te = ts + L.tile_words < L.words ? ts + L.tile_words : L.words;
+    uint64_t ws, we;
+    Tile::wave_range(ts, te, wave, ws, we);
+    unsigned long long *cursor = P.cursors + static_cast<size_t>(tile) * WAVES + wave;
     if (P.resume) {
         ws = *cursor;
     }
-
     // Skip the whole tile when every wave has already finished it.
     if (tid == 0) {
         s_live = 0;
@@ -256,218 +584,29 @@ __global__ __launch_bounds__(WAVES * 64) void sieve_kernel(
     if (!s_live) {
         return;
     }
-
-    // Prime selection (src/sieve_plan.cpp:75-86): the 32 largest candidates
-    // that do not divide b.  Lane i tests candidate i; its slot is the number
-    // of qualifying candidates above it.
-    if (wave == 0) {
-        const bool cand = lane < kCandidates;
-        const uint32_t p = cand ? static_cast<uint32_t>(P.primes[lane]) : 1u;
-        const uint32_t u = cand ? static_cast<uint32_t>(b) % p : 0u;
-        const bool ok = cand && u != 0;
-        const unsigned long long mask = __ballot(ok);
-        if (ok) {
-            const int rank = __popcll(mask >> (lane + 1));
-            if (rank < kSelected) {
-                s_p[rank] = p;
-                s_c[rank] = static_cast<uint32_t>(P.neg_bound_mod[lane]);
-                s_u[rank] = u;
-                s_idx[rank] = lane;
-                s_magic[rank] = static_cast<uint32_t>(0x100000000ULL / p);
-                s_base[rank] = static_cast<uint32_t>(ts % p);
-            }
-        }
-    }
-    __syncthreads();
-    for (int k = tid; k < kSelected * kResidWords; k += kThreads) {
-        const int s = k / kResidWords;
-        const int w = k - s * kResidWords;
-        cext[s * 10 + w] = P.resid[(static_cast<size_t>(s_idx[s]) * kMaxPrime
-                                    + s_u[s]) * kResidWords + w];
-    }
-    __syncthreads();
-    // Dense rows: row s entry e is word e of the line mod p, i.e. the window
-    // of the residue string at (-A + 64 e) mod p; entries e >= p repeat the
-    // row so that reads never wrap.
-    for (int s = 0; s < FAST; ++s) {
-        const uint32_t p = s_p[s];
-        const uint32_t c = s_c[s];
-        const uint32_t magic = s_magic[s];
-        const uint64_t *cx = cext + s * 10;
-        const int n = static_cast<int>(p) + kStride - 1;
-        for (int e = tid; e < n; e += kThreads) {
-            const uint32_t o = mod_small(c + 64u * static_cast<uint32_t>(e), p, magic);
-            rows[s * kSlot + e] = window64(cx, o);
-        }
-    }
-    __syncthreads();
-
-    if (ws >= we) {
-        if (lane == 0) {
-            *cursor = we;
-        }
-        return;
-    }
-
-    // Wave-uniform row offsets: word ws mod p for each dense prime.
-    uint32_t off[FAST];
-    uint32_t prime[FAST];
-    {
-        const uint32_t rel = static_cast<uint32_t>(ws - ts);
-        uint32_t mine = 0;
-        if (lane < FAST) {
-            const uint32_t p = s_p[lane];
-            uint32_t v = mod_small(rel, p, s_magic[lane]) + s_base[lane];
-            mine = v >= p ? v - p : v;
-        }
-#pragma unroll
-        for (int s = 0; s < FAST; ++s) {
-            off[s] = __builtin_amdgcn_readlane(mine, s);
-            prime[s] = uniform(s_p[s]);
-        }
-    }
-
-    // Late-prime constants: lane j of each 32-lane half owns prime FAST + j.
-    constexpr int kLate = kSelected - FAST;
-    static_assert(kLate <= 32, "late primes must fit a half wave");
-    const int lj = lane & 31;
-    const bool late_lane = lj < kLate;
-    uint32_t lp = 1, lmagic = 0, lbase = 0, lc = 0;
-    const uint64_t *lcx = cext;
-    if (late_lane) {
-        lp = s_p[FAST + lj];
-        lmagic = s_magic[FAST + lj];
-        lbase = s_base[FAST + lj];
-        lc = s_c[FAST + lj];
-        lcx = cext + (FAST + lj) * 10;
-    }
-
-    const uint64_t last_word = P.words - 1;
-    bool stopped = false;
-    uint64_t W = ws;
-    for (; W < we; W += kStride) {
-        uint64_t m[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            m[u] = ~0ULL;
-        }
-#pragma unroll
-        for (int s = 0; s < FAST; ++s) {
-            const uint64_t *r = rows + s * kSlot + off[s] + lane;
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                m[u] &= r[64 * u];
-            }
-            const uint32_t next = off[s] + kStride;
-            off[s] = next >= prime[s] ? next - prime[s] : next;
-        }
-        if (W + kStride >= we) {
-            // Last iteration of this wave: drop words past the wave's range
-            // and the sites past +A in the line's final word.
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint64_t w = W + lane + 64 * u;
-                if (w >= we) {
-                    m[u] = 0;
-                } else if (w == last_word) {
-                    m[u] &= P.last_mask;
-                }
-            }
-        }
-        uint64_t any = 0;
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            any |= m[u];
-        }
-        if (!__ballot(any != 0)) {
-            continue;
-        }
-        // Remaining primes, only for words that still have bits set.  Up to
-        // two such words are taken per pass as wave-uniform values; in each
-        // 32-lane half, lane j evaluates late prime FAST + j for its word and
-        // the half ANDs the row words together.
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            uint64_t pend = __ballot(m[u] != 0);
-            while (pend) {
-                const int l0 = __builtin_ctzll(pend);
-                pend &= pend - 1;
-                int l1 = -1;
-                if (pend) {
-                    l1 = __builtin_ctzll(pend);
-                    pend &= pend - 1;
-                }
-                const int src = lane < 32 ? l0 : l1;
-                uint64_t e = ~0ULL;
-                if (late_lane && src >= 0) {
-                    const uint32_t rel =
-                        static_cast<uint32_t>(W - ts) + src + 64u * u;
-                    uint32_t wm = mod_small(rel, lp, lmagic) + lbase;
-                    wm = wm >= lp ? wm - lp : wm;
-                    e = window64(lcx, mod_small(lc + 64u * wm, lp, lmagic));
-                }
-#pragma unroll
-                for (int d = 1; d < 32; d <<= 1) {
-                    e &= __shfl_xor(e, d, 64);
-                }
-                const uint64_t r0 = readlane64(e, 0);
-                const uint64_t r1 = readlane64(e, 32);
-                if (lane == l0) {
-                    m[u] &= r0;
-                }
-                if (lane == l1) {
-                    m[u] &= r1;
-                }
-            }
-        }
-        uint64_t live = 0;
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            live |= m[u];
-        }
-        if (!__ballot(live != 0)) {
-            continue;
-        }
-        if (!emit_survivors<U>(P, m, W + lane, b, lane)) {
-            stopped = true;
-            break;
-        }
-    }
+    Tile::setup(pool, b, ts, L, P.primes, tid, lane, wave);
+    const uint64_t stop = ws < we ? Tile::sweep(pool, b, ts, ws, we, L, P.out, 0, lane) : we;
     if (lane == 0) {
-        *cursor = stopped ? W : we;
+        *cursor = stop;
     }
 }
 
-// Short lines (few words per denominator): no row expansion.  A tile is a
-// block of up to NB denominators; the workgroup stages their residue strings
-// and prime parameters in LDS, and each lane takes one (denominator, word)
-// pair and ANDs the window of every selected prime directly, one Barrett
-// reduction and one funnel shift per prime, with a wave-uniform early exit
-// once every lane's word is empty.
 template <int WAVES, int NB>
-__global__ __launch_bounds__(WAVES * 64) void direct_kernel(
-    const SieveParams P) {
-    constexpr int kThreads = WAVES * 64;
-    __shared__ uint64_t cext[NB * kSelected * 10];
-    __shared__ uint32_t prm[NB * kSelected * 2];  // {p | c << 16, magic}
+__global__ __launch_bounds__(WAVES * 64) void direct_kernel(const SieveParams P) {
+    using Tile = DirectTile<WAVES, NB>;
+    __shared__ uint64_t pool[Tile::kLdsWords];
     __shared__ int s_live;
-
+    const LineDesc &L = P.line;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = static_cast<int>(uniform(tid >> 6));
     const int tile = blockIdx.x;
-    const int t0 = P.first_t + tile * P.tile_t;
-    const int nb = P.n_t - tile * P.tile_t < P.tile_t ? P.n_t - tile * P.tile_t : P.tile_t;
-    const uint32_t words = static_cast<uint32_t>(P.words);
-    const uint32_t total = static_cast<uint32_t>(nb) * words;
-    const uint32_t per = ((total + 64u * WAVES - 1) / (64u * WAVES)) * 64u;
-    uint32_t qs = per * wave;
-    const uint32_t qe = qs + per < total ? qs + per : total;
-    if (qs > qe) {
-        qs = qe;
-    }
-    unsigned long long *cursor =
-        P.cursors + static_cast<size_t>(tile) * WAVES + wave;
+    const int t0 = L.first_t + tile * L.tile_t;
+    const int nb = L.n_t - tile * L.tile_t < L.tile_t ? L.n_t - tile * L.tile_t : L.tile_t;
+    const uint32_t total = static_cast<uint32_t>(nb) * static_cast<uint32_t>(L.words);
+    uint32_t qs, qe;
+    Tile::wave_range(total, wave, qs, qe);
+    unsigned long long *cursor = P.cursors + static_cast<size_t>(tile) * WAVES + wave;
     if (P.resume) {
         qs = static_cast<uint32_t>(*cursor);
     }
@@ -482,73 +621,104 @@ __global__ __launch_bounds__(WAVES * 64) void direct_kernel(
     if (!s_live) {
         return;
     }
-    // Prime selection per denominator (src/sieve_plan.cpp:75-86), one wave
-    // per denominator; the lane that ranks a prime stages its string.
-    for (int bl = wave; bl < nb; bl += WAVES) {
-        const long long t = static_cast<long long>(t0) + bl;
-        const uint32_t b = static_cast<uint32_t>(P.square ? t * t : t);
-        const bool cand = lane < kCandidates;
-        const uint32_t p = cand ? static_cast<uint32_t>(P.primes[lane]) : 1u;
-        const uint32_t u = cand ? b % p : 0u;
-        const bool ok = cand && u != 0;
-        const unsigned long long mask = __ballot(ok);
-        if (ok) {
-            const int rank = __popcll(mask >> (lane + 1));
-            if (rank < kSelected) {
-                const int slot = bl * kSelected + rank;
-                prm[2 * slot] = p | (static_cast<uint32_t>(P.neg_bound_mod[lane]) << 16);
-                prm[2 * slot + 1] = static_cast<uint32_t>(0x100000000ULL / p);
-                const unsigned long long *src =
-                    P.resid + (static_cast<size_t>(lane) * kMaxPrime + u) * kResidWords;
-#pragma unroll
-                for (int w = 0; w < kResidWords; ++w) {
-                    cext[slot * 10 + w] = src[w];
-                }
-            }
-        }
+    Tile::setup(pool, t0, nb, L, P.primes, lane, wave);
+    const uint32_t stop = qs < qe ? Tile::sweep(pool, t0, qs, qe, L, P.out, 0, lane) : qe;
+    if (lane == 0) {
+        *cursor = stop;
     }
-    __syncthreads();
+}
 
-    bool stopped = false;
-    uint32_t Q = qs;
-    for (; Q < qe; Q += 64) {
-        const uint32_t q = Q + lane;
-        const bool valid = q < qe;
-        uint32_t bl = __umulhi(q, P.magic_words);
-        uint32_t w = q - bl * words;
-        if (w >= words) {
-            w -= words;
-            ++bl;
-        }
-        if (!valid) {
-            bl = 0;
-            w = 0;
-        }
-        uint64_t m[1];
-        m[0] = !valid ? 0ULL : (w == words - 1 ? P.last_mask : ~0ULL);
-        const uint32_t *pp = prm + bl * kSelected * 2;
-        const uint64_t *cx = cext + bl * kSelected * 10;
-        for (int s = 0; s < kSelected; ++s) {
-            if (s >= 8 && (s & 3) == 0 && !__ballot(m[0] != 0)) {
-                break;
+// ---------------------------------------------------------------------------
+// Persistent multi-curve launch
+// ---------------------------------------------------------------------------
+
+// Workgroups pull tile indices from M.head until all tiles are handed out or
+// the survivor buffer is full.  A tile whose survivors do not fit marks its
+// curve in curve_failed; later tiles of a marked curve are skipped (the host
+// searches such curves again on the plain path).  Tiles of all curves share
+// one index space, curves in the order the host chose.
+template <int U, int FAST, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4))) void multi_kernel(
+    const MultiParams M) {
+    using Row = RowTile<U, FAST, WAVES>;
+    using Direct = DirectTile<WAVES, kDirectMaxDenominators>;
+    constexpr int kWords = Row::kLdsWords > Direct::kLdsWords ? Row::kLdsWords
+                                                               : Direct::kLdsWords;
+    __shared__ uint64_t pool[kWords];
+    __shared__ unsigned long long s_tile;
+    __shared__ int s_curve;
+    __shared__ int s_skip;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = static_cast<int>(uniform(tid >> 6));
+    for (;;) {
+        if (tid == 0) {
+            unsigned long long t = ~0ULL;
+            if (__hip_atomic_load(&M.out.state->full, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                t = atomicAdd(M.head, 1ULL);
             }
-            const uint32_t pc = pp[2 * s];
-            const uint32_t p = pc & 0xffffu;
-            const uint32_t o = mod_small((pc >> 16) + 64u * w, p, pp[2 * s + 1]);
-            m[0] &= window64(cx + s * 10, o);
+            int c = 0;
+            int skip = 1;
+            if (t < M.total_tiles) {
+                int lo = 0, hi = M.n_curves;  // tile_start[lo] <= t < tile_start[hi]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (M.tile_start[mid] <= t) {
+                        lo = mid;
+                    } else {
+                        hi = mid;
+                    }
+                }
+                c = lo;
+                skip = __hip_atomic_load(&M.curve_failed[c], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+            }
+            s_tile = t;
+            s_curve = c;
+            s_skip = skip;
         }
-        if (!__ballot(m[0] != 0)) {
-            continue;
-        }
-        const long long t = static_cast<long long>(t0) + bl;
-        const int b = static_cast<int>(P.square ? t * t : t);
-        if (!emit_survivors<1>(P, m, w, b, lane)) {
-            stopped = true;
+        __syncthreads();
+        const unsigned long long tile = uniform64(s_tile);
+        const int c = static_cast<int>(uniform(static_cast<uint32_t>(s_curve)));
+        const int skip = static_cast<int>(uniform(static_cast<uint32_t>(s_skip)));
+        if (tile >= M.total_tiles) {
             break;
         }
-    }
-    if (lane == 0) {
-        *cursor = stopped ? Q : qe;
+        if (!skip) {
+            const CurveDesc &C = M.curves[c];
+            const LineDesc &L = C.line;
+            const unsigned long long local = tile - C.tile_start;
+            bool failed = false;
+            if (L.direct) {
+                const int t0 = L.first_t + static_cast<int>(local) * L.tile_t;
+                const int rest = L.n_t - static_cast<int>(local) * L.tile_t;
+                const int nb = rest < L.tile_t ? rest : L.tile_t;
+                uint32_t qs, qe;
+                Direct::wave_range(static_cast<uint32_t>(nb) * static_cast<uint32_t>(L.words),
+                                   wave, qs, qe);
+                Direct::setup(pool, t0, nb, L, M.primes, lane, wave);
+                if (qs < qe) {
+                    failed = Direct::sweep(pool, t0, qs, qe, L, M.out, C.request, lane) != qe;
+                }
+            } else {
+                const int t_index = static_cast<int>(local / static_cast<unsigned>(L.tiles_per_t));
+                const int chunk = static_cast<int>(local) - t_index * L.tiles_per_t;
+                const int b = denominator(L, static_cast<long long>(L.first_t) + t_index);
+                const uint64_t ts = static_cast<uint64_t>(chunk) * L.tile_words;
+                const uint64_t te = ts + L.tile_words < L.words ? ts + L.tile_words : L.words;
+                uint64_t ws, we;
+                Row::wave_range(ts, te, wave, ws, we);
+                Row::setup(pool, b, ts, L, M.primes, tid, lane, wave);
+                if (ws < we) {
+                    failed = Row::sweep(pool, b, ts, ws, we, L, M.out, C.request, lane) != we;
+                }
+            }
+            if (failed && lane == 0) {
+                atomicOr(&M.curve_failed[c], 1);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -562,16 +732,30 @@ hipError_t launch_variant(const SieveParams &params, unsigned grid,
     return hipGetLastError();
 }
 
+template <int U, int FAST, int WAVES>
+hipError_t launch_multi_variant(const MultiParams &params, unsigned grid,
+                                hipStream_t stream) {
+    hipLaunchKernelGGL((multi_kernel<U, FAST, WAVES>), dim3(grid),
+                       dim3(WAVES * 64), 0, stream, params);
+    return hipGetLastError();
+}
+
+#define RPS_VARIANT(U, F, W)                                               \
+    {"u" #U "_f" #F "_w" #W, U, F, W, launch_variant<U, F, W>,            \
+     launch_multi_variant<U, F, W>}
+
 const VariantInfo kVariants[] = {
-    {"u2_f14_w8", 2, 14, 8, launch_variant<2, 14, 8>},
-    {"u1_f14_w8", 1, 14, 8, launch_variant<1, 14, 8>},
-    {"u4_f14_w8", 4, 14, 8, launch_variant<4, 14, 8>},
-    {"u2_f12_w8", 2, 12, 8, launch_variant<2, 12, 8>},
-    {"u2_f16_w8", 2, 16, 8, launch_variant<2, 16, 8>},
-    {"u2_f14_w16", 2, 14, 16, launch_variant<2, 14, 16>},
-    {"u4_f14_w16", 4, 14, 16, launch_variant<4, 14, 16>},
-    {"u4_f16_w16", 4, 16, 16, launch_variant<4, 16, 16>},
+    RPS_VARIANT(2, 14, 8),
+    RPS_VARIANT(1, 14, 8),
+    RPS_VARIANT(4, 14, 8),
+    RPS_VARIANT(2, 12, 8),
+    RPS_VARIANT(2, 16, 8),
+    RPS_VARIANT(2, 14, 16),
+    RPS_VARIANT(4, 14, 16),
+    RPS_VARIANT(4, 16, 16),
 };
+
+#undef RPS_VARIANT
 
 }  // namespace
 
@@ -586,16 +770,18 @@ const VariantInfo *variant(int index) {
     return &kVariants[index];
 }
 
-hipError_t launch_chi(const int *coef_res, int n_coef, const int *primes,
-                      unsigned long long *chi, hipStream_t stream) {
-    hipLaunchKernelGGL(chi_kernel, dim3(kCandidates), dim3(512), 0, stream,
+hipError_t launch_chi(const int *coef_res, int n_coef, int curves,
+                      const int *primes, unsigned long long *chi,
+                      hipStream_t stream) {
+    hipLaunchKernelGGL(chi_kernel, dim3(kCandidates, curves), dim3(512), 0, stream,
                        coef_res, n_coef, primes, chi);
     return hipGetLastError();
 }
 
 hipError_t launch_resid(const int *primes, const unsigned long long *chi,
-                        unsigned long long *resid, hipStream_t stream) {
-    hipLaunchKernelGGL(resid_kernel, dim3(kCandidates, kMaxPrime / 4),
+                        int curves, unsigned long long *resid,
+                        hipStream_t stream) {
+    hipLaunchKernelGGL(resid_kernel, dim3(kCandidates, kMaxPrime / 4, curves),
                        dim3(256), 0, stream, primes, chi, resid);
     return hipGetLastError();
 }

@@ -8,7 +8,7 @@
 
 namespace rps {
 
-// A compiled instantiation of the sieve kernel.
+// A compiled instantiation of the row-tile sieve (plain and multi-curve).
 struct VariantInfo {
     const char *name;
     int words_per_lane;  // U: a wave covers 64 U words per iteration
@@ -16,15 +16,21 @@ struct VariantInfo {
     int waves;           // waves per workgroup
     hipError_t (*launch)(const SieveParams &params, unsigned grid,
                          hipStream_t stream);
+    hipError_t (*launch_multi)(const MultiParams &params, unsigned grid,
+                               hipStream_t stream);
 };
 
 int variant_count();
 const VariantInfo *variant(int index);
 
-hipError_t launch_chi(const int *coef_res, int n_coef, const int *primes,
-                      unsigned long long *chi, hipStream_t stream);
+// chi_p bitmaps and residue strings of `curves` curves; coefficient
+// residues are [curves][36][n_coef], zero-padded above each degree.
+hipError_t launch_chi(const int *coef_res, int n_coef, int curves,
+                      const int *primes, unsigned long long *chi,
+                      hipStream_t stream);
 hipError_t launch_resid(const int *primes, const unsigned long long *chi,
-                        unsigned long long *resid, hipStream_t stream);
+                        int curves, unsigned long long *resid,
+                        hipStream_t stream);
 hipError_t launch_reset(DeviceState *state, hipStream_t stream);
 
 // Short-line kernel: tiles of up to kDirectMaxDenominators whole lines of at

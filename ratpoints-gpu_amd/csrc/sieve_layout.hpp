@@ -15,10 +15,11 @@
 //     u = b mod p, the 576-bit string E(z) = chi_p((z mod p) * u^-1 mod p),
 //     z = 0..575 (nine 64-bit words).  A 64-bit window of E starting at bit
 //     (-A + 64 w) mod p is exactly word w of the line for denominator b;
-//   * the sieve kernel copies the strings of the 32 selected primes of its
+//   * a row tile copies the strings of the 32 selected primes of its
 //     denominator into LDS and expands the dense primes into rows of
 //     p + 64 U - 1 words, so a wave reads 64 U consecutive row words with a
-//     wave-uniform start and never wraps.
+//     wave-uniform start and never wraps; a short-line tile keeps only the
+//     strings and evaluates windows directly.
 #ifndef RATPOINTS_GPU_AMD_SIEVE_LAYOUT_HPP
 #define RATPOINTS_GPU_AMD_SIEVE_LAYOUT_HPP
 
@@ -33,6 +34,9 @@ constexpr int kChiWords = 8;        // 512-bit chi_p bitmap per prime
 constexpr int kResidWords = 9;      // 576-bit residue string per (p, u)
 constexpr int kMaxCoefficients = 101;  // degree <= 100
 constexpr int kMaxParamT = 46340;   // largest t with t^2 < 2^31
+// Residue-table words per curve: [36][512][9].
+constexpr unsigned long long kResidTableWords =
+    static_cast<unsigned long long>(kCandidates) * kMaxPrime * kResidWords;
 
 // Flush state of one survivor buffer.  Reservations are contiguous ranges
 // handed out by an atomic add on `count`; a wave whose range does not fit
@@ -45,27 +49,61 @@ struct DeviceState {
     unsigned int pad;
 };
 
-struct SieveParams {
-    long long bound;                 // A
-    unsigned long long words;        // 64-bit words per denominator line
-    unsigned long long last_mask;    // valid bits of the line's last word
-    unsigned long long tile_words;   // words per tile (a chunk of one line)
-    unsigned long long capacity;     // survivor slots in out_num / out_den
-    int first_t;                     // first denominator parameter
-    int n_t;                         // parameters in this launch
-    int tiles_per_t;                 // chunks per denominator line
-    int square;                      // b = t^2
-    int resume;                      // start every wave at its cursor
-    int tile_t;                      // denominators per short-line tile
-    unsigned magic_words;            // floor(2^32 / words), short lines
+// One curve's search: its lines, tables and tiling.
+struct LineDesc {
+    long long bound;                     // A
+    unsigned long long words;            // 64-bit words per line
+    unsigned long long last_mask;        // valid bits of the line's last word
+    unsigned long long tile_words;       // words per row tile (line chunk)
+    const unsigned long long *resid;     // [36][512][9] residue strings
+    const int *neg_bound_mod;            // [36] (-A) mod p
+    int first_t;                         // first denominator parameter
+    int n_t;                             // denominator parameters
+    int square;                          // b = t^2
+    int direct;                          // short lines: direct tiles
+    int tiles_per_t;                     // row tiles per denominator
+    int tile_t;                          // denominators per direct tile
+    unsigned magic_words;                // floor(2^32 / words), direct tiles
     int pad;
-    int primes[kCandidates];         // ascending
-    int neg_bound_mod[kCandidates];  // (-A) mod p
-    const unsigned long long *resid; // [36][512][9] residue strings
-    long long *out_num;
-    int *out_den;
+};
+
+// Where survivors go.
+struct OutDesc {
+    long long *num;
+    int *den;
+    int *req;                            // request index per survivor, or null
     DeviceState *state;
-    unsigned long long *cursors;     // [tiles][waves] next word per wave
+    unsigned long long capacity;
+};
+
+// Plain launch: one curve, one denominator batch, one workgroup per tile.
+struct SieveParams {
+    LineDesc line;
+    OutDesc out;
+    const int *primes;                   // [36] ascending
+    unsigned long long *cursors;         // [tiles][waves] next word per wave
+    int resume;                          // start every wave at its cursor
+    int pad;
+};
+
+// Persistent multi-curve launch: workgroups pull tiles from `head`.
+struct CurveDesc {
+    LineDesc line;
+    unsigned long long tile_start;       // first tile of this curve
+    int request;                         // index reported with survivors
+    int pad;
+};
+
+struct MultiParams {
+    const CurveDesc *curves;             // [n_curves], in tile order
+    const unsigned long long *tile_start;  // [n_curves + 1] tile prefix
+    int *curve_failed;                   // [n_curves] survivor reservation failed
+    unsigned long long *head;            // next tile to hand out
+    unsigned long long total_tiles;
+    OutDesc out;
+    const int *primes;
+    int n_curves;
+    int pad;
 };
 
 }  // namespace rps

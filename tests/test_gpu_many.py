@@ -1,0 +1,130 @@
+"""The persistent many-curve launch (rps_sieve_many) through the C ABI.
+
+Bar: for every request, the survivor multiset equals what rps_sieve gives
+for that request alone, and the CPU oracle's where the oracle is fast; a
+request whose survivors overflow the shared buffer is searched again alone
+and still comes out exact.
+"""
+
+import random
+
+import numpy as np
+import pytest
+
+import curves
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(**case):
+    print("case:", " ".join(f"{k}={v}" for k, v in case.items()), flush=True)
+
+
+def random_requests(seed, count, max_bound=40000, max_den=300):
+    rng = random.Random(seed)
+    reqs = []
+    for _ in range(count):
+        degree = rng.choice([1, 2, 3, 4, 5, 6, 8, 10])
+        coeffs, _ = curves.partially_dense(rng, rng.randint(6, 12), degree,
+                                           coeff_bound=rng.choice([10, 10 ** 9, 10 ** 40]))
+        square = rng.random() < 0.15
+        bound = rng.choice([1, 63, 64, 65, rng.randint(1, 3000), rng.randint(1, max_bound)])
+        if square:
+            first = rng.randint(1, 46000)
+            last = min(46340, first + rng.randint(0, max_den // 4))
+        else:
+            first = rng.choice([1, rng.randint(1, 10 ** 6), 2 ** 31 - 500])
+            last = min(2 ** 31 - 1, first + rng.randint(0, max_den))
+        reqs.append({"coefficients": coeffs, "bound": bound, "first": first, "last": last,
+                     "square": square})
+    return reqs
+
+
+def plain(ctx, r):
+    res = ctx.sieve(r["coefficients"], r["bound"], r["first"], r["last"], r.get("square", False))
+    return res.numerators, res.denominators
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_many_matches_plain_and_oracle(fresh_gpu, oracle_lib, seed):
+    reqs = random_requests(seed, 40)
+    announce(case="many", seed=seed, n=len(reqs))
+    results, stats = fresh_gpu.sieve_many(reqs)
+    assert stats["requeued"] == 0
+    total = 0
+    for r, (num, den) in zip(reqs, results):
+        pn, pd = plain(fresh_gpu, r)
+        assert np.array_equal(num, pn) and np.array_equal(den, pd)
+        if (2 * r["bound"] + 1) * (r["last"] - r["first"] + 1) < 3e7:
+            count, digest, on, od = oracle_lib.sieve(r["coefficients"], r["bound"], r["first"],
+                                                     r["last"], r.get("square", False),
+                                                     method="words")
+            assert np.array_equal(num, on) and np.array_equal(den, od)
+        total += len(num)
+    assert stats["survivor_count"] == total
+
+
+def test_many_long_lines_and_chunks(fresh_gpu):
+    # Lines longer than 2^18 words are cut into several tiles per line.
+    rng = random.Random(4)
+    reqs = []
+    for _ in range(6):
+        coeffs, _ = curves.partially_dense(rng, 14, rng.choice([3, 4, 6]))
+        reqs.append({"coefficients": coeffs, "bound": rng.randint(9_000_000, 20_000_000),
+                     "first": rng.randint(1, 1000), "last": 0})
+        reqs[-1]["last"] = reqs[-1]["first"] + rng.randint(0, 40)
+    announce(case="many-long", n=len(reqs))
+    results, stats = fresh_gpu.sieve_many(reqs)
+    for r, (num, den) in zip(reqs, results):
+        pn, pd = plain(fresh_gpu, r)
+        assert np.array_equal(num, pn) and np.array_equal(den, pd)
+
+
+def test_many_overflow_requeues_exactly(fresh_gpu, oracle_lib):
+    # Dense requests overflow a 16,384-survivor buffer: they are searched
+    # again alone; every request still matches the oracle.
+    fresh_gpu.set_option("survivor_capacity", 16384)
+    dense = ["1", str(curves.dense_product())]
+    reqs = random_requests(11, 12, max_bound=3000, max_den=60)
+    reqs.insert(3, {"coefficients": dense, "bound": 20000, "first": 1, "last": 2})
+    reqs.insert(8, {"coefficients": dense, "bound": 300, "first": 1, "last": 100})
+    announce(case="many-overflow", n=len(reqs))
+    results, stats = fresh_gpu.sieve_many(reqs)
+    assert stats["requeued"] >= 1
+    for r, (num, den) in zip(reqs, results):
+        count, digest, on, od = oracle_lib.sieve(r["coefficients"], r["bound"], r["first"],
+                                                 r["last"], r.get("square", False),
+                                                 method="words")
+        assert np.array_equal(num, on) and np.array_equal(den, od)
+
+
+def test_many_config3_sample(fresh_gpu, workloads):
+    # The batched workload's curves (bounds capped to keep the test short):
+    # the shared launch equals the per-curve plain loop, and every planted
+    # point survives.
+    sample = workloads.config3_curves(count=96)
+    for c in sample:
+        c["bound"] = min(c["bound"], 30000)
+        c["last"] = min(c["last"], 2000)
+        c["points"] = [(a, b) for a, b in c["points"] if abs(a) <= c["bound"] and b <= c["last"]]
+    announce(case="config3-sample", n=len(sample))
+    results, stats = fresh_gpu.sieve_many(sample)
+    for c, (num, den) in zip(sample, results):
+        pn, pd = plain(fresh_gpu, c)
+        assert np.array_equal(num, pn) and np.array_equal(den, pd)
+        got = set(zip(num.tolist(), den.tolist()))
+        assert all(p in got for p in c["points"])
+
+
+def test_many_cancel_and_errors(fresh_gpu, pkg):
+    reqs = random_requests(21, 5)
+    calls = []
+    rc, stats = fresh_gpu.run_many(reqs, lambda q, n, d: calls.append(len(n)) or True)
+    assert rc == 4 or not calls
+    bad = list(reqs) + [{"coefficients": ["1", "1"], "bound": 10, "first": 5, "last": 4}]
+    with pytest.raises(pkg.SieveError) as err:
+        fresh_gpu.run_many(bad)
+    assert err.value.code == 1
+    rc, stats = fresh_gpu.run_many([])
+    assert rc == 0 and stats["survivor_count"] == 0
