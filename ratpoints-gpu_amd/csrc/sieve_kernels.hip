@@ -59,25 +59,33 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
     return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
+// Orders one wave's LDS accesses before and after this point (lanes read
+// what other lanes of the wave wrote).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ int denominator(const LineDesc &L, long long t) {
     return static_cast<int>(L.square ? t * t : t);
 }
 
-// Wave-level survivor emission.  Lane l holds U masks; mask u covers the 64
-// numerators of word first_word + 64 u on denominator b.  A prefix sum of
-// per-lane counts gives every lane its slots under one atomic reservation per
-// wave, all or nothing: when the reservation does not fit, first_fail records
-// its start, nothing is written and false is returned, so the wave can stop
-// and resume at the same iteration after the host drains the buffer.
-template <int U>
-__device__ __forceinline__ bool emit_survivors(const OutDesc &O, long long bound,
-                                               const uint64_t (&m)[U],
-                                               uint64_t first_word, int b,
-                                               int request, int lane) {
+// Wave-level survivor emission.  Lane l holds N masks; mask k covers the 64
+// numerators of word word[k] on denominator b.  A prefix sum of per-lane
+// counts gives every lane its slots under one atomic reservation per wave,
+// all or nothing: when the reservation does not fit, first_fail records its
+// start, nothing is written and false is returned, so the wave can stop and
+// resume at the same point after the host drains the buffer.
+template <int N>
+__device__ __forceinline__ bool emit_words(const OutDesc &O, long long bound,
+                                           const uint64_t (&m)[N],
+                                           const uint64_t (&word)[N], int b,
+                                           int request, int lane) {
     uint32_t cnt = 0;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        cnt += __popcll(m[u]);
+    for (int k = 0; k < N; ++k) {
+        cnt += __popcll(m[k]);
     }
     uint32_t incl = cnt;
 #pragma unroll
@@ -103,9 +111,9 @@ __device__ __forceinline__ bool emit_survivors(const OutDesc &O, long long bound
     unsigned long long pos = start + incl - cnt;
     const uint64_t neg_a = static_cast<uint64_t>(-bound);
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        uint64_t mm = m[u];
-        const uint64_t a0 = neg_a + 64 * (first_word + 64 * u);
+    for (int k = 0; k < N; ++k) {
+        uint64_t mm = m[k];
+        const uint64_t a0 = neg_a + 64 * word[k];
         while (mm) {
             const int j = __ffsll(static_cast<long long>(mm)) - 1;
             mm &= mm - 1;
@@ -118,6 +126,20 @@ __device__ __forceinline__ bool emit_survivors(const OutDesc &O, long long bound
         }
     }
     return true;
+}
+
+// emit_words for U masks on the consecutive words first_word + 64 u.
+template <int U>
+__device__ __forceinline__ bool emit_survivors(const OutDesc &O, long long bound,
+                                               const uint64_t (&m)[U],
+                                               uint64_t first_word, int b,
+                                               int request, int lane) {
+    uint64_t word[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        word[u] = first_word + 64 * u;
+    }
+    return emit_words<U>(O, bound, m, word, b, request, lane);
 }
 
 }  // namespace
@@ -215,11 +237,59 @@ struct RowTile {
         __syncthreads();
     }
 
+    // Dense stage of one iteration: m[u] = AND over the FAST rows of entry
+    // off[s] + lane + 64 u, then every offset advances one stride (mod p).
+    // Rows are combined two at a time on 32-bit halves with the three-input
+    // AND (v_bitop3_b32), one instruction per half for two rows.
+    __device__ static void dense(const uint64_t *pool, uint32_t (&off)[FAST],
+                                 const uint32_t (&prime)[FAST], int lane,
+                                 uint64_t (&m)[U]) {
+        uint32_t lo[U], hi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            lo[u] = ~0u;
+            hi[u] = ~0u;
+        }
+#pragma unroll
+        for (int s = 0; s < FAST; s += 2) {
+            const uint64_t *r0 = pool + s * kSlot + off[s] + lane;
+            if (s + 1 < FAST) {
+                const uint64_t *r1 = pool + (s + 1) * kSlot + off[s + 1] + lane;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint64_t x = r0[64 * u];
+                    const uint64_t y = r1[64 * u];
+                    lo[u] = __builtin_amdgcn_bitop3_b32(lo[u], static_cast<uint32_t>(x),
+                                                        static_cast<uint32_t>(y), 0x80);
+                    hi[u] = __builtin_amdgcn_bitop3_b32(hi[u], static_cast<uint32_t>(x >> 32),
+                                                        static_cast<uint32_t>(y >> 32), 0x80);
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint64_t x = r0[64 * u];
+                    lo[u] &= static_cast<uint32_t>(x);
+                    hi[u] &= static_cast<uint32_t>(x >> 32);
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < FAST; ++s) {
+            const uint32_t next = off[s] + kStride;
+            off[s] = next >= prime[s] ? next - prime[s] : next;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            m[u] = (static_cast<uint64_t>(hi[u]) << 32) | lo[u];
+        }
+    }
+
     // One wave sweeps [ws, we).  Returns we when done, or the start of the
     // iteration whose survivors did not fit the buffer.
-    __device__ static uint64_t sweep(const uint64_t *pool, int b, uint64_t ts,
-                                     uint64_t ws, uint64_t we, const LineDesc &L,
-                                     const OutDesc &O, int request, int lane) {
+    __device__ static uint64_t sweep(uint64_t *pool, int b, uint64_t ts,
+                                     uint64_t ws, uint64_t we, int /*wave*/,
+                                     const LineDesc &L, const OutDesc &O,
+                                     int request, int lane) {
         const uint32_t *s_p = reinterpret_cast<const uint32_t *>(pool + kSmall);
         const uint32_t *s_c = s_p + kSelected;
         const uint32_t *s_base = s_c + 3 * kSelected;
@@ -260,20 +330,7 @@ struct RowTile {
         uint64_t W = ws;
         for (; W < we; W += kStride) {
             uint64_t m[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                m[u] = ~0ULL;
-            }
-#pragma unroll
-            for (int s = 0; s < FAST; ++s) {
-                const uint64_t *r = pool + s * kSlot + off[s] + lane;
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    m[u] &= r[64 * u];
-                }
-                const uint32_t next = off[s] + kStride;
-                off[s] = next >= prime[s] ? next - prime[s] : next;
-            }
+            dense(pool, off, prime, lane, m);
             if (W + kStride >= we) {
                 // Last iteration of this wave: drop words past the wave's
                 // range and the sites past +A in the line's final word.
@@ -344,6 +401,175 @@ struct RowTile {
             if (!emit_survivors<U>(O, L.bound, m, W + lane, b, request, lane)) {
                 return W;
             }
+        }
+        return we;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Row tile with a compaction queue for the late primes.  The dense stage is
+// RowTile's; words that still have bits set are appended (ballot + mbcnt)
+// to a per-wave queue of 128 entries in LDS, and whenever 64 or more are
+// queued the wave drains them with every lane busy -- up to two entries per
+// lane, each late prime evaluated from the residue string, a wave-uniform
+// exit once all entries are empty -- and commits all of them in one
+// reservation.  The resume cursor is W + u (iteration W, sub-word u), the
+// oldest uncommitted queue entry, so nothing is emitted twice.
+// ---------------------------------------------------------------------------
+template <int U, int FAST, int WAVES>
+struct RowTileQ {
+    using Base = RowTile<U, FAST, WAVES>;
+    static constexpr int kStride = Base::kStride;
+    static constexpr int kSlot = Base::kSlot;
+    static constexpr int kQueue = 128;
+    // Per wave: 128 masks (128 words) then 128 uint32 word offsets (64 words).
+    static constexpr int kQueueWords = kQueue + kQueue / 2;
+    static constexpr int kQueueBase = Base::kLdsWords;
+    static constexpr int kLdsWords = kQueueBase + WAVES * kQueueWords;
+    static_assert((kStride & (kStride - 1)) == 0, "the cursor encodes u below the stride");
+
+    __device__ static void wave_range(uint64_t ts, uint64_t te, int wave,
+                                      uint64_t &ws, uint64_t &we) {
+        Base::wave_range(ts, te, wave, ws, we);
+    }
+
+    __device__ static void setup(uint64_t *pool, int b, uint64_t ts,
+                                 const LineDesc &L, const int *primes,
+                                 int tid, int lane, int wave) {
+        Base::setup(pool, b, ts, L, primes, tid, lane, wave);
+    }
+
+    // Applies the late primes to the n queued entries (n <= 128) and emits
+    // the survivors; false when they did not fit the buffer.
+    __device__ static bool drain(const uint64_t *pool, const uint64_t *qm,
+                                 const uint32_t *qw, uint32_t n, int b,
+                                 uint64_t ts, const LineDesc &L,
+                                 const OutDesc &O, int request, int lane) {
+        const uint32_t *s_p = reinterpret_cast<const uint32_t *>(pool + Base::kSmall);
+        const uint32_t *s_c = s_p + kSelected;
+        const uint32_t *s_base = s_c + 3 * kSelected;
+        const uint32_t *s_magic = s_base + kSelected;
+        const uint64_t *cext = pool + Base::kCext;
+        uint64_t m[2];
+        uint32_t rel[2];
+        wave_sync();
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const uint32_t idx = lane + 64u * k;
+            const bool valid = idx < n;
+            m[k] = valid ? qm[idx] : 0ULL;
+            rel[k] = valid ? qw[idx] : 0u;
+        }
+        wave_sync();
+        for (int s = FAST; s < kSelected; ++s) {
+            if (!__ballot((m[0] | m[1]) != 0)) {
+                break;
+            }
+            const uint32_t p = uniform(s_p[s]);
+            const uint32_t magic = uniform(s_magic[s]);
+            const uint32_t base = uniform(s_base[s]);
+            const uint32_t c = uniform(s_c[s]);
+            const uint64_t *cx = cext + s * 10;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                if (m[k]) {
+                    uint32_t wm = mod_small(rel[k], p, magic) + base;
+                    wm = wm >= p ? wm - p : wm;
+                    m[k] &= window64(cx, mod_small(c + 64u * wm, p, magic));
+                }
+            }
+        }
+        if (!__ballot((m[0] | m[1]) != 0)) {
+            return true;
+        }
+        const uint64_t word[2] = {ts + rel[0], ts + rel[1]};
+        // The outcome is wave-uniform; saying so keeps the caller's loop
+        // (and its row offsets) in scalar registers.
+        return uniform(emit_words<2>(O, L.bound, m, word, b, request, lane) ? 1u : 0u) != 0;
+    }
+
+    // One wave sweeps [ws, we); `from` is the resume cursor (W + u) or ws.
+    // Returns we when done, or the cursor of the oldest entry that did not
+    // fit the buffer.
+    __device__ static uint64_t sweep(uint64_t *pool, int b, uint64_t ts,
+                                     uint64_t from, uint64_t we, int wave,
+                                     const LineDesc &L, const OutDesc &O,
+                                     int request, int lane) {
+        const uint32_t *s_p = reinterpret_cast<const uint32_t *>(pool + Base::kSmall);
+        const uint32_t *s_base = s_p + 4 * kSelected;
+        const uint32_t *s_magic = s_base + kSelected;
+        uint64_t *qm = pool + kQueueBase + wave * kQueueWords;
+        uint32_t *qw = reinterpret_cast<uint32_t *>(qm + kQueue);
+        // Iterations start at ts + k kStride, so the sub-word is the cursor's
+        // offset from the tile start modulo the stride.
+        const uint32_t skip0 = static_cast<uint32_t>((from - ts) & (kStride - 1));
+        const uint64_t ws = from - skip0;
+        uint32_t off[FAST];
+        uint32_t prime[FAST];
+        {
+            const uint32_t rel = static_cast<uint32_t>(ws - ts);
+            uint32_t mine = 0;
+            if (lane < FAST) {
+                const uint32_t p = s_p[lane];
+                const uint32_t v = mod_small(rel, p, s_magic[lane]) + s_base[lane];
+                mine = v >= p ? v - p : v;
+            }
+#pragma unroll
+            for (int s = 0; s < FAST; ++s) {
+                off[s] = __builtin_amdgcn_readlane(mine, s);
+                prime[s] = uniform(s_p[s]);
+            }
+        }
+        const uint64_t last_word = L.words - 1;
+        uint32_t queued = 0;
+        uint64_t oldest = ws;
+        uint32_t skip = skip0;
+        for (uint64_t W = ws; W < we; W += kStride) {
+            uint64_t m[U];
+            Base::dense(pool, off, prime, lane, m);
+            if (W + kStride >= we) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint64_t w = W + lane + 64 * u;
+                    if (w >= we) {
+                        m[u] = 0;
+                    } else if (w == last_word) {
+                        m[u] &= L.last_mask;
+                    }
+                }
+            }
+            const uint32_t rel = static_cast<uint32_t>(W - ts) + lane;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (static_cast<uint32_t>(u) < skip) {
+                    continue;
+                }
+                const uint64_t bal = __ballot(m[u] != 0);
+                if (!bal) {
+                    continue;
+                }
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                    static_cast<uint32_t>(bal >> 32),
+                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bal), 0u));
+                if (queued == 0) {
+                    oldest = W + u;
+                }
+                if (m[u]) {
+                    qm[queued + rank] = m[u];
+                    qw[queued + rank] = rel + 64u * u;
+                }
+                queued += static_cast<uint32_t>(__popcll(bal));
+                if (queued >= 64) {
+                    if (!drain(pool, qm, qw, queued, b, ts, L, O, request, lane)) {
+                        return oldest;
+                    }
+                    queued = 0;
+                }
+            }
+            skip = 0;
+        }
+        if (queued && !drain(pool, qm, qw, queued, b, ts, L, O, request, lane)) {
+            return oldest;
         }
         return we;
     }
@@ -551,9 +777,8 @@ __global__ void reset_state_kernel(DeviceState *state) {
 // Plain launches (one curve, one denominator batch)
 // ---------------------------------------------------------------------------
 
-template <int U, int FAST, int WAVES>
+template <class Tile, int WAVES>
 __global__ __launch_bounds__(WAVES * 64) void sieve_kernel(const SieveParams P) {
-    using Tile = RowTile<U, FAST, WAVES>;
     __shared__ uint64_t pool[Tile::kLdsWords];
     __shared__ int s_live;
     const LineDesc &L = P.line;
@@ -585,7 +810,7 @@ __global__ __launch_bounds__(WAVES * 64) void sieve_kernel(const SieveParams P) 
         return;
     }
     Tile::setup(pool, b, ts, L, P.primes, tid, lane, wave);
-    const uint64_t stop = ws < we ? Tile::sweep(pool, b, ts, ws, we, L, P.out, 0, lane) : we;
+    const uint64_t stop = ws < we ? Tile::sweep(pool, b, ts, ws, we, wave, L, P.out, 0, lane) : we;
     if (lane == 0) {
         *cursor = stop;
     }
@@ -637,10 +862,9 @@ __global__ __launch_bounds__(WAVES * 64) void direct_kernel(const SieveParams P)
 // curve in curve_failed; later tiles of a marked curve are skipped (the host
 // searches such curves again on the plain path).  Tiles of all curves share
 // one index space, curves in the order the host chose.
-template <int U, int FAST, int WAVES>
+template <class Row, int WAVES>
 __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4))) void multi_kernel(
     const MultiParams M) {
-    using Row = RowTile<U, FAST, WAVES>;
     using Direct = DirectTile<WAVES, kDirectMaxDenominators>;
     constexpr int kWords = Row::kLdsWords > Direct::kLdsWords ? Row::kLdsWords
                                                                : Direct::kLdsWords;
@@ -711,7 +935,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4)))
                 Row::wave_range(ts, te, wave, ws, we);
                 Row::setup(pool, b, ts, L, M.primes, tid, lane, wave);
                 if (ws < we) {
-                    failed = Row::sweep(pool, b, ts, ws, we, L, M.out, C.request, lane) != we;
+                    failed = Row::sweep(pool, b, ts, ws, we, wave, L, M.out, C.request, lane) != we;
                 }
             }
             if (failed && lane == 0) {
@@ -724,25 +948,31 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4)))
 
 namespace {
 
-template <int U, int FAST, int WAVES>
+template <class Tile, int WAVES>
 hipError_t launch_variant(const SieveParams &params, unsigned grid,
                           hipStream_t stream) {
-    hipLaunchKernelGGL((sieve_kernel<U, FAST, WAVES>), dim3(grid),
-                       dim3(WAVES * 64), 0, stream, params);
+    hipLaunchKernelGGL((sieve_kernel<Tile, WAVES>), dim3(grid), dim3(WAVES * 64), 0,
+                       stream, params);
     return hipGetLastError();
 }
 
-template <int U, int FAST, int WAVES>
+template <class Tile, int WAVES>
 hipError_t launch_multi_variant(const MultiParams &params, unsigned grid,
                                 hipStream_t stream) {
-    hipLaunchKernelGGL((multi_kernel<U, FAST, WAVES>), dim3(grid),
-                       dim3(WAVES * 64), 0, stream, params);
+    hipLaunchKernelGGL((multi_kernel<Tile, WAVES>), dim3(grid), dim3(WAVES * 64), 0,
+                       stream, params);
     return hipGetLastError();
 }
 
+// "u2_f14_w8": RowTile with U = 2, FAST = 14, 8 waves; "q_u2_f12_w8": the
+// same shape with the compaction-queue late stage.
 #define RPS_VARIANT(U, F, W)                                               \
-    {"u" #U "_f" #F "_w" #W, U, F, W, launch_variant<U, F, W>,            \
-     launch_multi_variant<U, F, W>}
+    {"u" #U "_f" #F "_w" #W, U, F, W, launch_variant<RowTile<U, F, W>, W>, \
+     launch_multi_variant<RowTile<U, F, W>, W>}
+#define RPS_VARIANT_Q(U, F, W)                                             \
+    {"q_u" #U "_f" #F "_w" #W, U, F, W,                                   \
+     launch_variant<RowTileQ<U, F, W>, W>,                                 \
+     launch_multi_variant<RowTileQ<U, F, W>, W>}
 
 const VariantInfo kVariants[] = {
     RPS_VARIANT(2, 14, 8),
@@ -753,9 +983,17 @@ const VariantInfo kVariants[] = {
     RPS_VARIANT(2, 14, 16),
     RPS_VARIANT(4, 14, 16),
     RPS_VARIANT(4, 16, 16),
+    RPS_VARIANT_Q(2, 10, 8),
+    RPS_VARIANT_Q(2, 11, 8),
+    RPS_VARIANT_Q(2, 12, 8),
+    RPS_VARIANT_Q(1, 12, 8),
+    RPS_VARIANT_Q(2, 12, 16),
+    RPS_VARIANT_Q(4, 11, 16),
+    RPS_VARIANT_Q(4, 12, 16),
 };
 
 #undef RPS_VARIANT
+#undef RPS_VARIANT_Q
 
 }  // namespace
 

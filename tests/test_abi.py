@@ -44,7 +44,7 @@ def test_variants_listed(pkg):
     names = pkg.variants()
     assert names and names[0] == "u2_f14_w8"
     for n in names:
-        assert re.fullmatch(r"u\d_f\d+_w\d+", n)
+        assert re.fullmatch(r"(q_)?u\d_f\d+_w\d+", n)
 
 
 def test_library_targets_gfx950(pkg):

@@ -81,15 +81,19 @@ def test_many_long_lines_and_chunks(fresh_gpu):
         assert np.array_equal(num, pn) and np.array_equal(den, pd)
 
 
-def test_many_overflow_requeues_exactly(fresh_gpu, oracle_lib):
+@pytest.mark.parametrize("variant", range(16))
+def test_many_overflow_requeues_exactly(fresh_gpu, oracle_lib, pkg, variant):
     # Dense requests overflow a 16,384-survivor buffer: they are searched
     # again alone; every request still matches the oracle.
+    if variant >= len(pkg.variants()):
+        pytest.skip("variant not compiled")
+    fresh_gpu.set_option("variant", variant)
     fresh_gpu.set_option("survivor_capacity", 16384)
     dense = ["1", str(curves.dense_product())]
     reqs = random_requests(11, 12, max_bound=3000, max_den=60)
     reqs.insert(3, {"coefficients": dense, "bound": 20000, "first": 1, "last": 2})
     reqs.insert(8, {"coefficients": dense, "bound": 300, "first": 1, "last": 100})
-    announce(case="many-overflow", n=len(reqs))
+    announce(case="many-overflow", variant=pkg.variants()[variant], n=len(reqs))
     results, stats = fresh_gpu.sieve_many(reqs)
     assert stats["requeued"] >= 1
     for r, (num, den) in zip(reqs, results):

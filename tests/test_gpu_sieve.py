@@ -119,7 +119,7 @@ def test_divisible_denominators(fresh_gpu, oracle_lib):
         assert_same(res, oracle_lib.sieve(coeffs, 30000, b, b + 2, method="words"))
 
 
-@pytest.mark.parametrize("variant", range(8))
+@pytest.mark.parametrize("variant", range(16))
 def test_variants_agree(fresh_gpu, oracle_lib, pkg, variant):
     if variant >= len(pkg.variants()):
         pytest.skip("variant not compiled")
@@ -128,6 +128,22 @@ def test_variants_agree(fresh_gpu, oracle_lib, pkg, variant):
     fresh_gpu.set_option("variant", variant)
     res = fresh_gpu.sieve(coeffs, 70001, 11, 140)
     assert_same(res, oracle_lib.sieve(coeffs, 70001, 11, 140, method="words"))
+
+
+@pytest.mark.parametrize("variant", range(16))
+def test_variants_resume_exactly_once(fresh_gpu, oracle_lib, pkg, variant):
+    # About a quarter of all sites survive (two live primes), so every wave
+    # overflows a 16,384-survivor buffer many times and resumes from its
+    # cursor; the multiset must still be exact.
+    if variant >= len(pkg.variants()):
+        pytest.skip("variant not compiled")
+    coeffs, _ = curves.partially_dense(random.Random(78 + variant), 2, 4)
+    announce(case="variant-resume", variant=pkg.variants()[variant], capacity=16384)
+    fresh_gpu.set_option("variant", variant)
+    fresh_gpu.set_option("survivor_capacity", 16384)
+    res = fresh_gpu.sieve(coeffs, 40000, 3, 22)
+    assert res.stats["launches"] >= res.survivor_count // 16384
+    assert_same(res, oracle_lib.sieve(coeffs, 40000, 3, 22, method="words"))
 
 
 def test_line_split_into_chunks(fresh_gpu, oracle_lib):
