@@ -336,7 +336,9 @@ unsigned long long line_chunks(long long min_tiles, long long n_t,
     }
     const unsigned long long min_words = 8ULL * stride * waves;
     chunks = std::min(chunks, std::max(1ULL, words / min_words));
-    const unsigned long long limit = 1ULL << 31;
+    // At most 2^25 words per tile: the kernels index words relative to the
+    // tile start in 32 bits and form 64 x index + offset without overflow.
+    const unsigned long long limit = 1ULL << 25;
     return std::max(chunks, (words + limit - 1) / limit);
 }
 

@@ -244,6 +244,16 @@ struct RowTile {
     __device__ static void dense(const uint64_t *pool, uint32_t (&off)[FAST],
                                  const uint32_t (&prime)[FAST], int lane,
                                  uint64_t (&m)[U]) {
+        // All loads first, so that each wave keeps FAST reads in flight.
+        uint64_t x[FAST][U];
+#pragma unroll
+        for (int s = 0; s < FAST; ++s) {
+            const uint64_t *r = pool + s * kSlot + off[s] + lane;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                x[s][u] = r[64 * u];
+            }
+        }
         uint32_t lo[U], hi[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -252,24 +262,18 @@ struct RowTile {
         }
 #pragma unroll
         for (int s = 0; s < FAST; s += 2) {
-            const uint64_t *r0 = pool + s * kSlot + off[s] + lane;
-            if (s + 1 < FAST) {
-                const uint64_t *r1 = pool + (s + 1) * kSlot + off[s + 1] + lane;
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint64_t x = r0[64 * u];
-                    const uint64_t y = r1[64 * u];
-                    lo[u] = __builtin_amdgcn_bitop3_b32(lo[u], static_cast<uint32_t>(x),
-                                                        static_cast<uint32_t>(y), 0x80);
-                    hi[u] = __builtin_amdgcn_bitop3_b32(hi[u], static_cast<uint32_t>(x >> 32),
-                                                        static_cast<uint32_t>(y >> 32), 0x80);
-                }
-            } else {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint64_t x = r0[64 * u];
-                    lo[u] &= static_cast<uint32_t>(x);
-                    hi[u] &= static_cast<uint32_t>(x >> 32);
+            for (int u = 0; u < U; ++u) {
+                if (s + 1 < FAST) {
+                    lo[u] = __builtin_amdgcn_bitop3_b32(lo[u], static_cast<uint32_t>(x[s][u]),
+                                                        static_cast<uint32_t>(x[s + 1][u]), 0x80);
+                    hi[u] = __builtin_amdgcn_bitop3_b32(hi[u],
+                                                        static_cast<uint32_t>(x[s][u] >> 32),
+                                                        static_cast<uint32_t>(x[s + 1][u] >> 32),
+                                                        0x80);
+                } else {
+                    lo[u] &= static_cast<uint32_t>(x[s][u]);
+                    hi[u] &= static_cast<uint32_t>(x[s][u] >> 32);
                 }
             }
         }
@@ -422,8 +426,9 @@ struct RowTileQ {
     static constexpr int kStride = Base::kStride;
     static constexpr int kSlot = Base::kSlot;
     static constexpr int kQueue = 128;
-    // Per wave: 128 masks (128 words) then 128 uint32 word offsets (64 words).
-    static constexpr int kQueueWords = kQueue + kQueue / 2;
+    // Per wave: 128 masks (128 words), 128 uint32 word offsets (64 words),
+    // then the late primes' line offsets (c + 64 (ts mod p)) mod p (16 words).
+    static constexpr int kQueueWords = kQueue + kQueue / 2 + kSelected / 2;
     static constexpr int kQueueBase = Base::kLdsWords;
     static constexpr int kLdsWords = kQueueBase + WAVES * kQueueWords;
     static_assert((kStride & (kStride - 1)) == 0, "the cursor encodes u below the stride");
@@ -440,16 +445,18 @@ struct RowTileQ {
     }
 
     // Applies the late primes to the n queued entries (n <= 128) and emits
-    // the survivors; false when they did not fit the buffer.
+    // the survivors; false when they did not fit the buffer.  Entry word
+    // ts + rel has offset (cb + 64 rel) mod p in the residue string (rel <
+    // 2^25, so the sum stays below 2^32), and its window is two funnel
+    // shifts of three 32-bit string words.
     __device__ static bool drain(const uint64_t *pool, const uint64_t *qm,
-                                 const uint32_t *qw, uint32_t n, int b,
-                                 uint64_t ts, const LineDesc &L,
-                                 const OutDesc &O, int request, int lane) {
+                                 const uint32_t *qw, const uint32_t *qcb,
+                                 uint32_t n, int b, uint64_t ts,
+                                 const LineDesc &L, const OutDesc &O,
+                                 int request, int lane) {
         const uint32_t *s_p = reinterpret_cast<const uint32_t *>(pool + Base::kSmall);
-        const uint32_t *s_c = s_p + kSelected;
-        const uint32_t *s_base = s_c + 3 * kSelected;
-        const uint32_t *s_magic = s_base + kSelected;
-        const uint64_t *cext = pool + Base::kCext;
+        const uint32_t *s_magic = s_p + 5 * kSelected;
+        const uint32_t *cext = reinterpret_cast<const uint32_t *>(pool + Base::kCext);
         uint64_t m[2];
         uint32_t rel[2];
         wave_sync();
@@ -458,7 +465,7 @@ struct RowTileQ {
             const uint32_t idx = lane + 64u * k;
             const bool valid = idx < n;
             m[k] = valid ? qm[idx] : 0ULL;
-            rel[k] = valid ? qw[idx] : 0u;
+            rel[k] = valid ? qw[idx] << 6 : 0u;
         }
         wave_sync();
         for (int s = FAST; s < kSelected; ++s) {
@@ -467,22 +474,26 @@ struct RowTileQ {
             }
             const uint32_t p = uniform(s_p[s]);
             const uint32_t magic = uniform(s_magic[s]);
-            const uint32_t base = uniform(s_base[s]);
-            const uint32_t c = uniform(s_c[s]);
-            const uint64_t *cx = cext + s * 10;
+            const uint32_t cb = uniform(qcb[s - FAST]);
+            const uint32_t *cx = cext + s * 20;
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 if (m[k]) {
-                    uint32_t wm = mod_small(rel[k], p, magic) + base;
-                    wm = wm >= p ? wm - p : wm;
-                    m[k] &= window64(cx, mod_small(c + 64u * wm, p, magic));
+                    const uint32_t o = mod_small(cb + rel[k], p, magic);
+                    const uint32_t *d = cx + (o >> 5);
+                    const uint32_t d0 = d[0];
+                    const uint32_t d1 = d[1];
+                    const uint32_t d2 = d[2];
+                    const uint32_t lo = __builtin_amdgcn_alignbit(d1, d0, o & 31);
+                    const uint32_t hi = __builtin_amdgcn_alignbit(d2, d1, o & 31);
+                    m[k] &= (static_cast<uint64_t>(hi) << 32) | lo;
                 }
             }
         }
         if (!__ballot((m[0] | m[1]) != 0)) {
             return true;
         }
-        const uint64_t word[2] = {ts + rel[0], ts + rel[1]};
+        const uint64_t word[2] = {ts + (rel[0] >> 6), ts + (rel[1] >> 6)};
         // The outcome is wave-uniform; saying so keeps the caller's loop
         // (and its row offsets) in scalar registers.
         return uniform(emit_words<2>(O, L.bound, m, word, b, request, lane) ? 1u : 0u) != 0;
@@ -500,6 +511,14 @@ struct RowTileQ {
         const uint32_t *s_magic = s_base + kSelected;
         uint64_t *qm = pool + kQueueBase + wave * kQueueWords;
         uint32_t *qw = reinterpret_cast<uint32_t *>(qm + kQueue);
+        uint32_t *qcb = qw + kQueue;
+        if (lane < kSelected - FAST) {
+            const uint32_t *s_c = s_p + kSelected;
+            const uint32_t p = s_p[FAST + lane];
+            qcb[lane] = mod_small(s_c[FAST + lane] + 64u * s_base[FAST + lane], p,
+                                  s_magic[FAST + lane]);
+        }
+        wave_sync();
         // Iterations start at ts + k kStride, so the sub-word is the cursor's
         // offset from the tile start modulo the stride.
         const uint32_t skip0 = static_cast<uint32_t>((from - ts) & (kStride - 1));
@@ -538,12 +557,28 @@ struct RowTileQ {
                     }
                 }
             }
+            if (skip) {
+                // First iteration of a resumed sweep: sub-words below the
+                // cursor were committed before.
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (static_cast<uint32_t>(u) < skip) {
+                        m[u] = 0;
+                    }
+                }
+                skip = 0;
+            }
+            uint64_t any = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                any |= m[u];
+            }
+            if (!__ballot(any != 0)) {
+                continue;
+            }
             const uint32_t rel = static_cast<uint32_t>(W - ts) + lane;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                if (static_cast<uint32_t>(u) < skip) {
-                    continue;
-                }
                 const uint64_t bal = __ballot(m[u] != 0);
                 if (!bal) {
                     continue;
@@ -560,15 +595,14 @@ struct RowTileQ {
                 }
                 queued += static_cast<uint32_t>(__popcll(bal));
                 if (queued >= 64) {
-                    if (!drain(pool, qm, qw, queued, b, ts, L, O, request, lane)) {
+                    if (!drain(pool, qm, qw, qcb, queued, b, ts, L, O, request, lane)) {
                         return oldest;
                     }
                     queued = 0;
                 }
             }
-            skip = 0;
         }
-        if (queued && !drain(pool, qm, qw, queued, b, ts, L, O, request, lane)) {
+        if (queued && !drain(pool, qm, qw, qcb, queued, b, ts, L, O, request, lane)) {
             return oldest;
         }
         return we;
