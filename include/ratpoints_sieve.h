@@ -161,7 +161,8 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests,
 int rps_set_option(rps_ctx *ctx, const char *name, int64_t value);
 
 /* Number of compiled kernel variants and a short name for each, e.g.
- * "u2_f14_w8" (words per lane, dense primes, waves per workgroup). */
+ * "q_u2_f12_w8" (q_: late primes through a compaction queue; words per lane,
+ * dense primes, waves per workgroup).  Variant 0 is the default. */
 int rps_variant_count(void);
 const char *rps_variant_name(int variant);
 

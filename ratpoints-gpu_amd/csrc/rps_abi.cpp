@@ -195,6 +195,7 @@ struct rps_ctx {
     Grow<int> m_negmod;
     Grow<rps::CurveDesc> m_curves;
     Grow<unsigned long long> m_tile_start;
+    Grow<int> m_bucket;
     Grow<int> m_failed;
     Grow<unsigned long long> m_head;
     Grow<int> m_req;
@@ -349,6 +350,20 @@ void row_tiling(rps::LineDesc &L, unsigned long long chunks, int stride) {
     L.direct = 0;
     L.tile_words = tile;
     L.tiles_per_t = static_cast<int>((L.words + tile - 1) / tile);
+    L.tile_t = 1;
+}
+
+// Whole-line row tiles of lines shorter than kRunWords take several
+// consecutive denominators (a run) so that per-tile work stays near
+// kRunWords words; at most max_run lines per tile.
+constexpr unsigned long long kRunWords = 1ULL << 17;
+
+void row_runs(rps::LineDesc &L, long long max_run) {
+    if (L.tiles_per_t != 1 || L.words >= kRunWords) {
+        return;
+    }
+    const long long want = static_cast<long long>(kRunWords / L.words);
+    L.tile_t = static_cast<int>(std::max(1LL, std::min(want, max_run)));
 }
 
 // Short lines: enough denominators per tile to give every wave of the
@@ -362,10 +377,8 @@ void direct_tiling(rps::LineDesc &L) {
 }
 
 size_t tile_count(const rps::LineDesc &L) {
-    if (L.direct) {
-        return (static_cast<size_t>(L.n_t) + L.tile_t - 1) / L.tile_t;
-    }
-    return static_cast<size_t>(L.n_t) * static_cast<size_t>(L.tiles_per_t);
+    const size_t runs = (static_cast<size_t>(L.n_t) + L.tile_t - 1) / L.tile_t;
+    return L.direct ? runs : runs * static_cast<size_t>(L.tiles_per_t);
 }
 
 constexpr unsigned long long kMaxTilesPerLaunch = 1ULL << 22;
@@ -474,6 +487,7 @@ void rps_close(rps_ctx *ctx) {
     ctx->m_negmod.release();
     ctx->m_curves.release();
     ctx->m_tile_start.release();
+    ctx->m_bucket.release();
     ctx->m_failed.release();
     ctx->m_head.release();
     ctx->m_req.release();
@@ -691,6 +705,7 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
                 n = std::min<long long>(n, static_cast<long long>(kMaxTilesPerLaunch / chunks));
                 chunks = line_chunks(ctx->min_tiles, n, words, stride, var->waves);
                 row_tiling(bt.line, chunks, stride);
+                row_runs(bt.line, std::max(1LL, n / std::max(1LL, ctx->min_tiles)));
             }
             bt.line.first_t = static_cast<int>(next_t);
             bt.line.n_t = static_cast<int>(n);
@@ -840,6 +855,7 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
                 const unsigned long long chunks = std::max(
                     (L.words + (1ULL << 18) - 1) >> 18, 1ULL);
                 row_tiling(L, chunks, stride);
+                row_runs(L, L.n_t);
             }
             lines[r] = L;
             const double line_sites = 2.0 * static_cast<double>(L.bound) + 1.0;
@@ -884,6 +900,26 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
             tile_start[k + 1] = tile_start[k] + tile_count(C.line);
         }
         const unsigned long long total_tiles = tile_start[n];
+        // Curve of every 2^shift-th tile (at most 2^20 buckets), so the
+        // kernel's search for a tile's curve spans only the curves of one
+        // bucket.
+        int bucket_shift = 0;
+        while ((total_tiles >> bucket_shift) > (1ULL << 20)) {
+            ++bucket_shift;
+        }
+        const size_t n_buckets = static_cast<size_t>(
+            ((total_tiles + (1ULL << bucket_shift) - 1) >> bucket_shift) + 1);
+        std::vector<int> bucket(n_buckets);
+        for (size_t g = 0, k = 0; g < n_buckets; ++g) {
+            const unsigned long long first = std::min<unsigned long long>(
+                static_cast<unsigned long long>(g) << bucket_shift,
+                total_tiles ? total_tiles - 1 : 0);
+            while (k + 1 < static_cast<size_t>(n) && tile_start[k + 1] <= first) {
+                ++k;
+            }
+            bucket[g] = static_cast<int>(k);
+        }
+        ctx->m_bucket.ensure(n_buckets);
         const auto t_workspace = Clock::now();
         st.plan_ms = ms_between(t_plan, t_workspace);
         RPS_HIP(hipMemcpy(ctx->m_coef.ptr, coef.data(), coef.size() * sizeof(int),
@@ -894,6 +930,8 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
                           hipMemcpyHostToDevice));
         RPS_HIP(hipMemcpy(ctx->m_tile_start.ptr, tile_start.data(),
                           (n + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice));
+        RPS_HIP(hipMemcpy(ctx->m_bucket.ptr, bucket.data(), n_buckets * sizeof(int),
+                          hipMemcpyHostToDevice));
         RPS_HIP(hipMemsetAsync(ctx->m_failed.ptr, 0, n * sizeof(int), ctx->compute));
         RPS_HIP(hipMemsetAsync(ctx->m_head.ptr, 0, sizeof(unsigned long long), ctx->compute));
         const auto t_basis = Clock::now();
@@ -918,6 +956,8 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
         rps::MultiParams M{};
         M.curves = ctx->m_curves.ptr;
         M.tile_start = ctx->m_tile_start.ptr;
+        M.bucket = ctx->m_bucket.ptr;
+        M.bucket_shift = bucket_shift;
         M.curve_failed = ctx->m_failed.ptr;
         M.head = ctx->m_head.ptr;
         M.total_tiles = total_tiles;

@@ -153,6 +153,8 @@ __device__ __forceinline__ bool emit_survivors(const OutDesc &O, long long bound
 // ---------------------------------------------------------------------------
 template <int U, int FAST, int WAVES>
 struct RowTile {
+    using Base = RowTile;
+    static constexpr int kWaves = WAVES;
     static constexpr int kThreads = WAVES * 64;
     static constexpr int kStride = 64 * U;
     static constexpr int kSlot = kMaxPrime + 64 * U;  // >= p + 64 U - 1 words
@@ -220,9 +222,19 @@ struct RowTile {
                                         + s_u[s]) * kResidWords + w];
         }
         __syncthreads();
-        // Dense rows: entry e of row s is word e of the line mod p, the window
-        // of the residue string at (-A + 64 e) mod p; entries e >= p repeat
-        // the row so that reads never wrap.
+        build_rows(pool, tid);
+        __syncthreads();
+    }
+
+    // Dense rows from the strings and selection in LDS: entry e of row s is
+    // word e of the line mod p, the window of the residue string at
+    // (-A + 64 e) mod p; entries e >= p repeat the row so that reads never
+    // wrap.
+    __device__ static void build_rows(uint64_t *pool, int tid) {
+        const uint32_t *s_p = reinterpret_cast<const uint32_t *>(pool + kSmall);
+        const uint32_t *s_c = s_p + kSelected;
+        const uint32_t *s_magic = s_p + 5 * kSelected;
+        const uint64_t *cext = pool + kCext;
         for (int s = 0; s < FAST; ++s) {
             const uint32_t p = s_p[s];
             const uint32_t c = s_c[s];
@@ -234,7 +246,6 @@ struct RowTile {
                 pool[s * kSlot + e] = window64(cx, o);
             }
         }
-        __syncthreads();
     }
 
     // Dense stage of one iteration: m[u] = AND over the FAST rows of entry
@@ -423,6 +434,7 @@ struct RowTile {
 template <int U, int FAST, int WAVES>
 struct RowTileQ {
     using Base = RowTile<U, FAST, WAVES>;
+    static constexpr int kWaves = WAVES;
     static constexpr int kStride = Base::kStride;
     static constexpr int kSlot = Base::kSlot;
     static constexpr int kQueue = 128;
@@ -606,6 +618,144 @@ struct RowTileQ {
             return oldest;
         }
         return we;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Runs of lines.  A row tile covers nr consecutive denominators of one curve
+// (nr > 1 only when the tile is the whole line).  The curve's candidate
+// constants stay in LDS for the tile; per line, wave 0 selects the primes,
+// the workgroup builds the dense rows, and the residue strings of the next
+// line are loaded into registers while the current line is swept, so a line
+// costs no global-memory round trip after the first.
+// ---------------------------------------------------------------------------
+template <class Row>
+struct RowRun {
+    using Base = typename Row::Base;
+    static constexpr int kThreads = Row::kWaves * 64;
+    // u32 arrays after the row tile's LDS: candidate p, magic and (-A) mod p
+    // [36 each], then the next line's selection: candidate index and b mod p
+    // [32 each].
+    static constexpr int kCand = Row::kLdsWords;
+    static constexpr int kLdsWords = kCand + (3 * kCandidates + 2 * kSelected + 1) / 2;
+    static_assert(kSelected * kResidWords <= kThreads, "one string word per thread");
+
+    // Cursor of a wave within a run: line index above bit 40, word below.
+    static constexpr int kLineShift = 40;
+    static constexpr uint64_t kPosMask = (1ULL << kLineShift) - 1;
+
+    __device__ static uint32_t *cand(uint64_t *pool) {
+        return reinterpret_cast<uint32_t *>(pool + kCand);
+    }
+
+    // Candidate constants of curve L (threads 0..35; the caller syncs).
+    __device__ static void load_curve(uint64_t *pool, const LineDesc &L,
+                                      const int *primes, int tid) {
+        uint32_t *cp = cand(pool);
+        if (tid < kCandidates) {
+            const uint32_t p = static_cast<uint32_t>(primes[tid]);
+            cp[tid] = p;
+            cp[kCandidates + tid] = 0xffffffffu / p;  // floor(2^32 / p), p odd
+            cp[2 * kCandidates + tid] = static_cast<uint32_t>(L.neg_bound_mod[tid]);
+        }
+    }
+
+    // src/sieve_plan.cpp:75-86: the 32 largest candidates not dividing b.
+    // Lane i tests candidate i; its slot is the number of qualifying
+    // candidates above it.  Returns the slot (or -1) and u = b mod p.
+    __device__ static int rank_of(const uint32_t *cp, uint32_t b, int lane,
+                                  uint32_t &p, uint32_t &magic, uint32_t &u) {
+        const bool cand_lane = lane < kCandidates;
+        p = cand_lane ? cp[lane] : 1u;
+        magic = cand_lane ? cp[kCandidates + lane] : 0u;
+        u = cand_lane ? mod_small(b, p, magic) : 0u;
+        const bool ok = cand_lane && u != 0;
+        const unsigned long long mask = __ballot(ok);
+        const int rank = ok ? __popcll(mask >> (lane + 1)) : kSelected;
+        return rank < kSelected ? rank : -1;
+    }
+
+    // Wave 0: the selection of denominator b for the line chunk at ts into
+    // the row tile's arrays.
+    __device__ static void select(uint64_t *pool, uint32_t b, uint64_t ts, int lane) {
+        uint32_t *cp = cand(pool);
+        uint32_t *s_p = reinterpret_cast<uint32_t *>(pool + Base::kSmall);
+        uint32_t p, magic, u;
+        const int rank = rank_of(cp, b, lane, p, magic, u);
+        if (rank >= 0) {
+            s_p[rank] = p;
+            s_p[kSelected + rank] = cp[2 * kCandidates + lane];  // c
+            s_p[2 * kSelected + rank] = u;
+            s_p[3 * kSelected + rank] = static_cast<uint32_t>(lane);
+            s_p[4 * kSelected + rank] = static_cast<uint32_t>(ts % p);
+            s_p[5 * kSelected + rank] = magic;
+        }
+    }
+
+    // Wave 0: candidate index and u of denominator b into the next arrays.
+    __device__ static void select_next(uint64_t *pool, uint32_t b, int lane) {
+        uint32_t *cp = cand(pool);
+        uint32_t *nx = cp + 3 * kCandidates;
+        uint32_t p, magic, u;
+        const int rank = rank_of(cp, b, lane, p, magic, u);
+        if (rank >= 0) {
+            nx[rank] = static_cast<uint32_t>(lane);
+            nx[kSelected + rank] = u;
+        }
+    }
+
+    // One string word of the next line per thread (threads below 288).
+    __device__ static uint64_t fetch(uint64_t *pool, const LineDesc &L, int tid) {
+        if (tid >= kSelected * kResidWords) {
+            return 0;
+        }
+        const uint32_t *nx = cand(pool) + 3 * kCandidates;
+        const int s = tid / kResidWords;
+        const int w = tid - s * kResidWords;
+        return L.resid[(static_cast<size_t>(nx[s]) * kMaxPrime + nx[kSelected + s])
+                       * kResidWords + w];
+    }
+
+    __device__ static void store(uint64_t *pool, uint64_t v, int tid) {
+        if (tid < kSelected * kResidWords) {
+            const int s = tid / kResidWords;
+            const int w = tid - s * kResidWords;
+            pool[Base::kCext + s * 10 + w] = v;
+        }
+    }
+
+    // Strings of denominator b into registers, before the first line.
+    __device__ static uint64_t begin(uint64_t *pool, const LineDesc &L, uint32_t b,
+                                     int tid, int lane, int wave) {
+        if (wave == 0) {
+            select_next(pool, b, lane);
+        }
+        __syncthreads();
+        return fetch(pool, L, tid);
+    }
+
+    // Set-up of the line of denominator b (its strings in v); when `more`,
+    // starts loading the strings of denominator bn and returns them.
+    // `flag`, when given, is read between the barriers (no sweep runs then,
+    // so every wave sees the same value) into `flag_value`.
+    __device__ static uint64_t setup_line(uint64_t *pool, const LineDesc &L, uint32_t b,
+                                          uint64_t ts, bool more, uint32_t bn,
+                                          uint64_t v, int tid, int lane, int wave,
+                                          const int *flag, int &flag_value) {
+        __syncthreads();  // the previous line's sweep is over
+        flag_value = flag ? static_cast<int>(uniform(static_cast<uint32_t>(*flag))) : 0;
+        store(pool, v, tid);
+        if (wave == 0) {
+            select(pool, b, ts, lane);
+            if (more) {
+                select_next(pool, bn, lane);
+            }
+        }
+        __syncthreads();
+        Base::build_rows(pool, tid);
+        const uint64_t nv = more ? fetch(pool, L, tid) : 0;
+        __syncthreads();
+        return nv;
     }
 };
 
@@ -811,42 +961,75 @@ __global__ void reset_state_kernel(DeviceState *state) {
 // Plain launches (one curve, one denominator batch)
 // ---------------------------------------------------------------------------
 
+// Tile = (run of L.tile_t denominators, chunk of the line).  Each wave keeps
+// a cursor (line of the run, position) so that a launch after a full buffer
+// resumes exactly where each wave stopped.
 template <class Tile, int WAVES>
-__global__ __launch_bounds__(WAVES * 64) void sieve_kernel(const SieveParams P) {
-    __shared__ uint64_t pool[Tile::kLdsWords];
-    __shared__ int s_live;
+__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4))) void sieve_kernel(
+    const SieveParams P) {
+    using Run = RowRun<Tile>;
+    __shared__ uint64_t pool[Run::kLdsWords];
+    __shared__ unsigned s_first;
     const LineDesc &L = P.line;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = static_cast<int>(uniform(tid >> 6));
     const int tile = blockIdx.x;
-    const int t_index = tile / L.tiles_per_t;
-    const int chunk = tile - t_index * L.tiles_per_t;
-    const int b = denominator(L, static_cast<long long>(L.first_t) + t_index);
+    const int run = tile / L.tiles_per_t;
+    const int chunk = tile - run * L.tiles_per_t;
+    const int t0 = L.first_t + run * L.tile_t;
+    const int rest = L.n_t - run * L.tile_t;
+    const int nr = rest < L.tile_t ? rest : L.tile_t;
     const uint64_t ts = static_cast<uint64_t>(chunk) * L.tile_words;
     const uint64_t te = ts + L.tile_words < L.words ? ts + L.tile_words : L.words;
     uint64_t ws, we;
     Tile::wave_range(ts, te, wave, ws, we);
     unsigned long long *cursor = P.cursors + static_cast<size_t>(tile) * WAVES + wave;
+    uint32_t cr = 0;
+    uint64_t cpos = ws;
     if (P.resume) {
-        ws = *cursor;
+        const uint64_t c = uniform64(*cursor);
+        cr = static_cast<uint32_t>(c >> Run::kLineShift);
+        cpos = c & Run::kPosMask;
     }
-    // Skip the whole tile when every wave has already finished it.
+    // Skip the tile when every wave has finished it; otherwise start at the
+    // earliest line any wave still needs.
     if (tid == 0) {
-        s_live = 0;
+        s_first = static_cast<unsigned>(nr);
+    }
+    Run::load_curve(pool, L, P.primes, tid);
+    __syncthreads();
+    if (lane == 0 && cr < static_cast<uint32_t>(nr)) {
+        atomicMin(&s_first, cr);
     }
     __syncthreads();
-    if (lane == 0 && ws < we) {
-        atomicOr(&s_live, 1);
-    }
-    __syncthreads();
-    if (!s_live) {
+    const int r0 = static_cast<int>(uniform(s_first));
+    if (r0 >= nr) {
         return;
     }
-    Tile::setup(pool, b, ts, L, P.primes, tid, lane, wave);
-    const uint64_t stop = ws < we ? Tile::sweep(pool, b, ts, ws, we, wave, L, P.out, 0, lane) : we;
+    uint64_t v = Run::begin(pool, L, static_cast<uint32_t>(denominator(L, t0 + r0)), tid,
+                            lane, wave);
+    bool failed = false;
+    for (int r = r0; r < nr; ++r) {
+        const bool more = r + 1 < nr;
+        const int b = denominator(L, static_cast<long long>(t0) + r);
+        const int bn = more ? denominator(L, static_cast<long long>(t0) + r + 1) : 0;
+        int unused;
+        v = Run::setup_line(pool, L, static_cast<uint32_t>(b), ts, more,
+                            static_cast<uint32_t>(bn), v, tid, lane, wave, nullptr, unused);
+        if (!failed && static_cast<uint32_t>(r) >= cr && ws < we) {
+            const uint64_t from = static_cast<uint32_t>(r) == cr ? cpos : ws;
+            const uint64_t stop = Tile::sweep(pool, b, ts, from, we, wave, L, P.out, 0, lane);
+            if (stop != we) {
+                failed = true;
+                cr = static_cast<uint32_t>(r);
+                cpos = stop;
+            }
+        }
+    }
     if (lane == 0) {
-        *cursor = stop;
+        *cursor = failed ? (static_cast<uint64_t>(cr) << Run::kLineShift) | cpos
+                         : static_cast<uint64_t>(nr) << Run::kLineShift;
     }
 }
 
@@ -893,19 +1076,23 @@ __global__ __launch_bounds__(WAVES * 64) void direct_kernel(const SieveParams P)
 
 // Workgroups pull tile indices from M.head until all tiles are handed out or
 // the survivor buffer is full.  A tile whose survivors do not fit marks its
-// curve in curve_failed; later tiles of a marked curve are skipped (the host
-// searches such curves again on the plain path).  Tiles of all curves share
-// one index space, curves in the order the host chose.
+// curve in curve_failed; the rest of that tile and later tiles of a marked
+// curve are skipped (the host searches such curves again on the plain
+// path).  Tiles of all curves share one index space, curves in the order the
+// host chose; bucket[g] is the curve holding tile g << bucket_shift, so a
+// tile's curve is found by a binary search over a few curves at most.
 template <class Row, int WAVES>
 __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4))) void multi_kernel(
     const MultiParams M) {
+    using Run = RowRun<Row>;
     using Direct = DirectTile<WAVES, kDirectMaxDenominators>;
-    constexpr int kWords = Row::kLdsWords > Direct::kLdsWords ? Row::kLdsWords
+    constexpr int kWords = Run::kLdsWords > Direct::kLdsWords ? Run::kLdsWords
                                                                : Direct::kLdsWords;
     __shared__ uint64_t pool[kWords];
     __shared__ unsigned long long s_tile;
     __shared__ int s_curve;
     __shared__ int s_skip;
+    __shared__ int s_fail;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = static_cast<int>(uniform(tid >> 6));
@@ -919,7 +1106,9 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4)))
             int c = 0;
             int skip = 1;
             if (t < M.total_tiles) {
-                int lo = 0, hi = M.n_curves;  // tile_start[lo] <= t < tile_start[hi]
+                const unsigned long long g = t >> M.bucket_shift;
+                int lo = M.bucket[g];
+                int hi = M.bucket[g + 1] + 1;  // tile_start[lo] <= t < tile_start[hi]
                 while (hi - lo > 1) {
                     const int mid = (lo + hi) >> 1;
                     if (M.tile_start[mid] <= t) {
@@ -935,6 +1124,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4)))
             s_tile = t;
             s_curve = c;
             s_skip = skip;
+            s_fail = 0;
         }
         __syncthreads();
         const unsigned long long tile = uniform64(s_tile);
@@ -960,16 +1150,37 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4)))
                     failed = Direct::sweep(pool, t0, qs, qe, L, M.out, C.request, lane) != qe;
                 }
             } else {
-                const int t_index = static_cast<int>(local / static_cast<unsigned>(L.tiles_per_t));
-                const int chunk = static_cast<int>(local) - t_index * L.tiles_per_t;
-                const int b = denominator(L, static_cast<long long>(L.first_t) + t_index);
+                const int run = static_cast<int>(local / static_cast<unsigned>(L.tiles_per_t));
+                const int chunk = static_cast<int>(local) - run * L.tiles_per_t;
+                const int t0 = L.first_t + run * L.tile_t;
+                const int rest = L.n_t - run * L.tile_t;
+                const int nr = rest < L.tile_t ? rest : L.tile_t;
                 const uint64_t ts = static_cast<uint64_t>(chunk) * L.tile_words;
                 const uint64_t te = ts + L.tile_words < L.words ? ts + L.tile_words : L.words;
                 uint64_t ws, we;
                 Row::wave_range(ts, te, wave, ws, we);
-                Row::setup(pool, b, ts, L, M.primes, tid, lane, wave);
-                if (ws < we) {
-                    failed = Row::sweep(pool, b, ts, ws, we, wave, L, M.out, C.request, lane) != we;
+                Run::load_curve(pool, L, M.primes, tid);
+                __syncthreads();
+                uint64_t v = Run::begin(pool, L, static_cast<uint32_t>(denominator(L, t0)), tid,
+                                        lane, wave);
+                for (int r = 0; r < nr; ++r) {
+                    const bool more = r + 1 < nr;
+                    const int b = denominator(L, static_cast<long long>(t0) + r);
+                    const int bn = more ? denominator(L, static_cast<long long>(t0) + r + 1) : 0;
+                    int stop_run;
+                    v = Run::setup_line(pool, L, static_cast<uint32_t>(b), ts, more,
+                                        static_cast<uint32_t>(bn), v, tid, lane, wave, &s_fail,
+                                        stop_run);
+                    if (stop_run) {
+                        break;
+                    }
+                    if (ws < we
+                        && Row::sweep(pool, b, ts, ws, we, wave, L, M.out, C.request, lane) != we) {
+                        failed = true;
+                        if (lane == 0) {
+                            s_fail = 1;
+                        }
+                    }
                 }
             }
             if (failed && lane == 0) {
@@ -1009,21 +1220,14 @@ hipError_t launch_multi_variant(const MultiParams &params, unsigned grid,
      launch_multi_variant<RowTileQ<U, F, W>, W>}
 
 const VariantInfo kVariants[] = {
-    RPS_VARIANT(2, 14, 8),
-    RPS_VARIANT(1, 14, 8),
-    RPS_VARIANT(4, 14, 8),
-    RPS_VARIANT(2, 12, 8),
-    RPS_VARIANT(2, 16, 8),
-    RPS_VARIANT(2, 14, 16),
-    RPS_VARIANT(4, 14, 16),
-    RPS_VARIANT(4, 16, 16),
-    RPS_VARIANT_Q(2, 10, 8),
-    RPS_VARIANT_Q(2, 11, 8),
     RPS_VARIANT_Q(2, 12, 8),
+    RPS_VARIANT_Q(2, 11, 8),
+    RPS_VARIANT_Q(2, 10, 8),
     RPS_VARIANT_Q(1, 12, 8),
     RPS_VARIANT_Q(2, 12, 16),
-    RPS_VARIANT_Q(4, 11, 16),
     RPS_VARIANT_Q(4, 12, 16),
+    RPS_VARIANT(2, 14, 8),
+    RPS_VARIANT(2, 14, 16),
 };
 
 #undef RPS_VARIANT

@@ -62,7 +62,7 @@ struct LineDesc {
     int square;                          // b = t^2
     int direct;                          // short lines: direct tiles
     int tiles_per_t;                     // row tiles per denominator
-    int tile_t;                          // denominators per direct tile
+    int tile_t;                          // denominators per tile (a run)
     unsigned magic_words;                // floor(2^32 / words), direct tiles
     int pad;
 };
@@ -97,13 +97,14 @@ struct CurveDesc {
 struct MultiParams {
     const CurveDesc *curves;             // [n_curves], in tile order
     const unsigned long long *tile_start;  // [n_curves + 1] tile prefix
+    const int *bucket;                   // [buckets + 1] curve of tile g << bucket_shift
     int *curve_failed;                   // [n_curves] survivor reservation failed
     unsigned long long *head;            // next tile to hand out
     unsigned long long total_tiles;
     OutDesc out;
     const int *primes;
     int n_curves;
-    int pad;
+    int bucket_shift;
 };
 
 }  // namespace rps

@@ -42,7 +42,7 @@ def test_library_exports_every_declared_symbol(pkg):
 
 def test_variants_listed(pkg):
     names = pkg.variants()
-    assert names and names[0] == "u2_f14_w8"
+    assert names and names[0] == "q_u2_f12_w8"
     for n in names:
         assert re.fullmatch(r"(q_)?u\d_f\d+_w\d+", n)
 

@@ -146,6 +146,43 @@ def test_variants_resume_exactly_once(fresh_gpu, oracle_lib, pkg, variant):
     assert_same(res, oracle_lib.sieve(coeffs, 40000, 3, 22, method="words"))
 
 
+@pytest.mark.parametrize("variant", range(16))
+def test_runs_of_lines(fresh_gpu, oracle_lib, pkg, variant):
+    # min_tiles = 1 lets lines shorter than 2^17 words share a tile: 2,501-word
+    # lines go 52 to a tile, so b = 7..140 is two full runs and a partial
+    # one, each line's strings loaded while the previous line is swept.
+    if variant >= len(pkg.variants()):
+        pytest.skip("variant not compiled")
+    coeffs, _ = curves.partially_dense(random.Random(90 + variant), 7, 6)
+    announce(case="runs", variant=pkg.variants()[variant])
+    fresh_gpu.set_option("variant", variant)
+    fresh_gpu.set_option("min_tiles", 1)
+    res = fresh_gpu.sieve(coeffs, 80000, 7, 140)
+    assert_same(res, oracle_lib.sieve(coeffs, 80000, 7, 140, method="words"))
+    # 94-word lines on row tiles (short-line kernel off): all 200 square
+    # denominators in one run.
+    fresh_gpu.set_option("direct_words", 0)
+    res = fresh_gpu.sieve(coeffs, 3000, 1, 200, square=True)
+    assert_same(res, oracle_lib.sieve(coeffs, 3000, 1, 200, square=True, method="words"))
+
+
+@pytest.mark.parametrize("variant", range(16))
+def test_runs_resume_exactly_once(fresh_gpu, oracle_lib, pkg, variant):
+    # Runs of 104 lines of 1,251 words with about a quarter of all sites
+    # surviving: waves stop mid-run on a full 16,384-survivor buffer and
+    # resume at their (line, word) cursors.
+    if variant >= len(pkg.variants()):
+        pytest.skip("variant not compiled")
+    coeffs, _ = curves.partially_dense(random.Random(120 + variant), 2, 5)
+    announce(case="runs-resume", variant=pkg.variants()[variant], capacity=16384)
+    fresh_gpu.set_option("variant", variant)
+    fresh_gpu.set_option("min_tiles", 1)
+    fresh_gpu.set_option("survivor_capacity", 16384)
+    res = fresh_gpu.sieve(coeffs, 40000, 300, 450)
+    assert res.stats["launches"] >= res.survivor_count // 16384
+    assert_same(res, oracle_lib.sieve(coeffs, 40000, 300, 450, method="words"))
+
+
 def test_line_split_into_chunks(fresh_gpu, oracle_lib):
     # Few denominators and a long line: each line is cut into many tiles.
     coeffs, _ = curves.partially_dense(random.Random(8), 9, 4)
