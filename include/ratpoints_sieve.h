@@ -157,6 +157,10 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests,
  *                        this many tiles per launch exist (default 2048)
  *   "direct_words"       lines of at most this many 64-bit words use the
  *                        short-line kernel, 0..768 (default 768; 0 = never)
+ *   "table_slots"        residue tables (1.33 MB each) kept on the device
+ *                        between rps_sieve_many calls, least recently used
+ *                        replaced, 1..65536 (default 256); a call with more
+ *                        distinct curves grows the cache to fit them
  */
 int rps_set_option(rps_ctx *ctx, const char *name, int64_t value);
 

@@ -15,6 +15,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -191,7 +192,16 @@ struct rps_ctx {
     // Many-curve launches.
     Grow<int> m_coef;
     Grow<unsigned long long> m_chi;
-    Grow<unsigned long long> m_resid;
+    // Residue tables of recent curves stay on the device between calls of
+    // rps_sieve_many; the least recently used are replaced.
+    Grow<unsigned long long> t_resid;  // [t_cap][36][512][9]
+    Grow<int> t_slot;                  // table slot of each curve being built
+    int t_cap = 0;
+    long long table_slots = 256;
+    std::map<std::vector<std::string>, int> t_index;
+    std::vector<std::vector<std::string>> t_key;  // per slot
+    std::vector<uint64_t> t_used;                 // per slot: last call
+    uint64_t t_call = 0;
     Grow<int> m_negmod;
     Grow<rps::CurveDesc> m_curves;
     Grow<unsigned long long> m_tile_start;
@@ -312,7 +322,8 @@ void prepare_curve(rps_ctx *ctx, const rps_request *req, rps_stats *st) {
                            hipMemcpyHostToDevice, ctx->compute));
     RPS_HIP(hipEventRecord(ctx->basis_start, ctx->compute));
     RPS_HIP(rps::launch_chi(ctx->d_coef_res, n, 1, ctx->d_primes, ctx->d_chi, ctx->compute));
-    RPS_HIP(rps::launch_resid(ctx->d_primes, ctx->d_chi, 1, ctx->d_resid, ctx->compute));
+    RPS_HIP(rps::launch_resid(ctx->d_primes, ctx->d_chi, 1, nullptr, ctx->d_resid,
+                              ctx->compute));
     RPS_HIP(hipEventRecord(ctx->basis_end, ctx->compute));
     RPS_HIP(hipEventSynchronize(ctx->basis_end));
     float ms = 0.0f;
@@ -326,8 +337,7 @@ void prepare_curve(rps_ctx *ctx, const rps_request *req, rps_stats *st) {
 
 // Number of word chunks each denominator line is split into.  Lines are
 // split only when a batch has too few denominators to fill the device, and
-// always into chunks of at most 2^31 words (the kernel's local word offsets
-// are 32-bit).  A chunk keeps at least eight strides per wave so the per-tile
+// always into chunks of at most 2^25 words.  A chunk keeps at least eight strides per wave so the per-tile
 // row build stays amortised.
 unsigned long long line_chunks(long long min_tiles, long long n_t,
                                unsigned long long words, int stride, int waves) {
@@ -483,7 +493,8 @@ void rps_close(rps_ctx *ctx) {
     if (ctx->h_req) (void)hipHostFree(ctx->h_req);
     ctx->m_coef.release();
     ctx->m_chi.release();
-    ctx->m_resid.release();
+    ctx->t_resid.release();
+    ctx->t_slot.release();
     ctx->m_negmod.release();
     ctx->m_curves.release();
     ctx->m_tile_start.release();
@@ -531,6 +542,12 @@ int rps_set_option(rps_ctx *ctx, const char *name, int64_t value) {
             return RPS_EINVAL;
         }
         ctx->direct_words = static_cast<unsigned long long>(value);
+    } else if (key == "table_slots") {
+        if (value < 1 || value > (1 << 16)) {
+            ctx->error = "table_slots must be in 1..65536";
+            return RPS_EINVAL;
+        }
+        ctx->table_slots = value;
     } else if (key == "min_tiles") {
         if (value < 1 || value > (1 << 22)) {
             ctx->error = "min_tiles must be in 1..4194304";
@@ -867,10 +884,79 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
         std::stable_sort(order.begin(), order.end(),
                          [&](int a, int b) { return cost[a] > cost[b]; });
 
-        // Per-curve tables, one batched build for all curves.
-        ctx->m_coef.ensure(static_cast<size_t>(n) * rps::kCandidates * n_coef);
-        ctx->m_chi.ensure(static_cast<size_t>(n) * rps::kCandidates * rps::kChiWords);
-        ctx->m_resid.ensure(static_cast<size_t>(n) * rps::kResidTableWords);
+        // Residue tables: one per distinct curve, kept between calls.  Hits
+        // are marked first; misses then take the least recently used slots
+        // not needed by this call.
+        ++ctx->t_call;
+        std::vector<std::vector<std::string>> keys(n);
+        std::map<std::vector<std::string>, int> distinct;
+        for (int k = 0; k < n; ++k) {
+            const rps_request &q = requests[order[k]];
+            keys[k].assign(q.coefficients, q.coefficients + q.n_coefficients);
+            distinct.emplace(keys[k], 0);
+        }
+        const int cap = static_cast<int>(std::max<long long>(
+            ctx->table_slots, static_cast<long long>(distinct.size())));
+        if (cap > ctx->t_cap) {
+            ctx->t_resid.release();
+            ctx->t_resid.ensure(static_cast<size_t>(cap) * rps::kResidTableWords);
+            ctx->t_cap = cap;
+            ctx->t_index.clear();
+            ctx->t_key.assign(cap, {});
+            ctx->t_used.assign(cap, 0);
+        }
+        for (auto &d : distinct) {
+            auto it = ctx->t_index.find(d.first);
+            d.second = -1;
+            if (it != ctx->t_index.end()) {
+                d.second = it->second;
+                ctx->t_used[it->second] = ctx->t_call;
+            }
+        }
+        std::vector<int> spare;
+        for (int slot = 0; slot < ctx->t_cap; ++slot) {
+            if (ctx->t_used[slot] != ctx->t_call) {
+                spare.push_back(slot);
+            }
+        }
+        std::stable_sort(spare.begin(), spare.end(), [&](int a, int b) {
+            return ctx->t_used[a] < ctx->t_used[b];
+        });
+        std::vector<int> slot_of(n), build_k, build_slot;
+        size_t next_spare = 0;
+        for (int k = 0; k < n; ++k) {
+            int &slot = distinct[keys[k]];
+            if (slot < 0) {
+                slot = spare.at(next_spare++);
+                if (!ctx->t_key[slot].empty()) {
+                    ctx->t_index.erase(ctx->t_key[slot]);
+                }
+                ctx->t_key[slot] = keys[k];
+                ctx->t_index[keys[k]] = slot;
+                ctx->t_used[slot] = ctx->t_call;
+                build_k.push_back(k);
+                build_slot.push_back(slot);
+            }
+            slot_of[k] = slot;
+        }
+        const int n_build = static_cast<int>(build_k.size());
+        // Until the new tables are built, a failure must not leave them in
+        // the cache.
+        struct TableGuard {
+            rps_ctx *ctx;
+            bool built = false;
+            ~TableGuard() {
+                if (!built) {
+                    ctx->t_index.clear();
+                    ctx->t_key.assign(ctx->t_cap, {});
+                    ctx->t_used.assign(ctx->t_cap, 0);
+                }
+            }
+        } guard{ctx};
+
+        ctx->m_coef.ensure(std::max<size_t>(1, static_cast<size_t>(n_build) * rps::kCandidates * n_coef));
+        ctx->m_chi.ensure(std::max<size_t>(1, static_cast<size_t>(n_build) * rps::kCandidates * rps::kChiWords));
+        ctx->t_slot.ensure(std::max(1, n_build));
         ctx->m_negmod.ensure(static_cast<size_t>(n) * rps::kCandidates);
         ctx->m_curves.ensure(n);
         ctx->m_tile_start.ensure(n + 1);
@@ -878,14 +964,16 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
         ctx->m_head.ensure(1);
         ctx->m_req.ensure(ctx->capacity);
         ensure_capacity(ctx);
-        std::vector<int> coef(static_cast<size_t>(n) * rps::kCandidates * n_coef);
+        std::vector<int> coef(static_cast<size_t>(n_build) * rps::kCandidates * n_coef);
+        for (int j = 0; j < n_build; ++j) {
+            coefficient_residues(&requests[order[build_k[j]]], ctx->primes, n_coef,
+                                 &coef[static_cast<size_t>(j) * rps::kCandidates * n_coef]);
+        }
         std::vector<int> negmod(static_cast<size_t>(n) * rps::kCandidates);
         std::vector<rps::CurveDesc> curves(n);
         std::vector<unsigned long long> tile_start(n + 1, 0);
         for (int k = 0; k < n; ++k) {
             const int r = order[k];
-            coefficient_residues(&requests[r], ctx->primes, n_coef,
-                                 &coef[static_cast<size_t>(k) * rps::kCandidates * n_coef]);
             for (int i = 0; i < rps::kCandidates; ++i) {
                 const long long p = ctx->primes[i];
                 negmod[static_cast<size_t>(k) * rps::kCandidates + i] =
@@ -893,7 +981,7 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
             }
             rps::CurveDesc &C = curves[k];
             C.line = lines[r];
-            C.line.resid = ctx->m_resid.ptr + static_cast<size_t>(k) * rps::kResidTableWords;
+            C.line.resid = ctx->t_resid.ptr + static_cast<size_t>(slot_of[k]) * rps::kResidTableWords;
             C.line.neg_bound_mod = ctx->m_negmod.ptr + static_cast<size_t>(k) * rps::kCandidates;
             C.request = r;
             C.tile_start = tile_start[k];
@@ -922,8 +1010,12 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
         ctx->m_bucket.ensure(n_buckets);
         const auto t_workspace = Clock::now();
         st.plan_ms = ms_between(t_plan, t_workspace);
-        RPS_HIP(hipMemcpy(ctx->m_coef.ptr, coef.data(), coef.size() * sizeof(int),
-                          hipMemcpyHostToDevice));
+        if (n_build) {
+            RPS_HIP(hipMemcpy(ctx->m_coef.ptr, coef.data(), coef.size() * sizeof(int),
+                              hipMemcpyHostToDevice));
+            RPS_HIP(hipMemcpy(ctx->t_slot.ptr, build_slot.data(), n_build * sizeof(int),
+                              hipMemcpyHostToDevice));
+        }
         RPS_HIP(hipMemcpy(ctx->m_negmod.ptr, negmod.data(), negmod.size() * sizeof(int),
                           hipMemcpyHostToDevice));
         RPS_HIP(hipMemcpy(ctx->m_curves.ptr, curves.data(), n * sizeof(rps::CurveDesc),
@@ -937,12 +1029,15 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
         const auto t_basis = Clock::now();
         st.workspace_ms = ms_between(t_workspace, t_basis);
         RPS_HIP(hipEventRecord(ctx->basis_start, ctx->compute));
-        RPS_HIP(rps::launch_chi(ctx->m_coef.ptr, n_coef, n, ctx->d_primes, ctx->m_chi.ptr,
-                                ctx->compute));
-        RPS_HIP(rps::launch_resid(ctx->d_primes, ctx->m_chi.ptr, n, ctx->m_resid.ptr,
-                                  ctx->compute));
+        if (n_build) {
+            RPS_HIP(rps::launch_chi(ctx->m_coef.ptr, n_coef, n_build, ctx->d_primes,
+                                    ctx->m_chi.ptr, ctx->compute));
+            RPS_HIP(rps::launch_resid(ctx->d_primes, ctx->m_chi.ptr, n_build, ctx->t_slot.ptr,
+                                      ctx->t_resid.ptr, ctx->compute));
+        }
         RPS_HIP(hipEventRecord(ctx->basis_end, ctx->compute));
         RPS_HIP(hipEventSynchronize(ctx->basis_end));
+        guard.built = true;
         float bms = 0.0f;
         RPS_HIP(hipEventElapsedTime(&bms, ctx->basis_start, ctx->basis_end));
         st.basis_ms = bms;

@@ -908,14 +908,16 @@ __global__ __launch_bounds__(512) void chi_kernel(
 __global__ __launch_bounds__(256) void resid_kernel(
     const int *__restrict__ primes,
     const unsigned long long *__restrict__ chi,
-    unsigned long long *__restrict__ resid) {  // [curves][36][512][9]
+    const int *__restrict__ slot,            // [curves] table slot, or null
+    unsigned long long *__restrict__ resid) {  // [slots][36][512][9]
     const int i = blockIdx.x;
     const size_t curve = blockIdx.z;
     const int lane = threadIdx.x & 63;
     const int u = blockIdx.y * 4 + (threadIdx.x >> 6);
     const uint32_t p = primes[i];
     const unsigned long long *ch = chi + (curve * kCandidates + i) * kChiWords;
-    unsigned long long *dst = resid + curve * kResidTableWords
+    const size_t table = slot ? static_cast<size_t>(slot[curve]) : curve;
+    unsigned long long *dst = resid + table * kResidTableWords
         + (static_cast<size_t>(i) * kMaxPrime + u) * kResidWords;
     if (u == 0 || u >= static_cast<int>(p)) {
         if (lane < kResidWords) {
@@ -1255,10 +1257,10 @@ hipError_t launch_chi(const int *coef_res, int n_coef, int curves,
 }
 
 hipError_t launch_resid(const int *primes, const unsigned long long *chi,
-                        int curves, unsigned long long *resid,
+                        int curves, const int *slot, unsigned long long *resid,
                         hipStream_t stream) {
     hipLaunchKernelGGL(resid_kernel, dim3(kCandidates, kMaxPrime / 4, curves),
-                       dim3(256), 0, stream, primes, chi, resid);
+                       dim3(256), 0, stream, primes, chi, slot, resid);
     return hipGetLastError();
 }
 

@@ -24,12 +24,14 @@ int variant_count();
 const VariantInfo *variant(int index);
 
 // chi_p bitmaps and residue strings of `curves` curves; coefficient
-// residues are [curves][36][n_coef], zero-padded above each degree.
+// residues are [curves][36][n_coef], zero-padded above each degree.  Curve
+// i's residue table goes to table slot[i] of `resid` (table i when slot is
+// null).
 hipError_t launch_chi(const int *coef_res, int n_coef, int curves,
                       const int *primes, unsigned long long *chi,
                       hipStream_t stream);
 hipError_t launch_resid(const int *primes, const unsigned long long *chi,
-                        int curves, unsigned long long *resid,
+                        int curves, const int *slot, unsigned long long *resid,
                         hipStream_t stream);
 hipError_t launch_reset(DeviceState *state, hipStream_t stream);
 

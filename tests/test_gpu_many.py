@@ -132,3 +132,19 @@ def test_many_cancel_and_errors(fresh_gpu, pkg):
     assert err.value.code == 1
     rc, stats = fresh_gpu.run_many([])
     assert rc == 0 and stats["survivor_count"] == 0
+
+
+def test_many_table_cache_reuse_and_eviction(pkg):
+    # Residue tables stay on the device between calls.  A fresh context with
+    # 8 table slots: the second call reuses 3 cached curves and builds 6 (the
+    # cache grows to the 9 distinct curves of one call), the third reuses
+    # some and rebuilds evicted ones; every result equals a plain search.
+    with pkg.SieveContext(0) as ctx:
+        ctx.set_option("table_slots", 8)
+        reqs = random_requests(31, 12)
+        for part in (reqs[:6], reqs[3:12], reqs[:4] + reqs[10:12], reqs[:6]):
+            announce(case="table-cache", n=len(part))
+            results, stats = ctx.sieve_many(part)
+            for r, (num, den) in zip(part, results):
+                pn, pd = plain(ctx, r)
+                assert np.array_equal(num, pn) and np.array_equal(den, pd)
