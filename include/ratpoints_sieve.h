@@ -126,6 +126,14 @@ typedef int (*rps_batch_done)(void *user, int32_t first_t, int32_t last_t,
 int rps_sieve_batched(rps_ctx *ctx, const rps_request *request, rps_sink sink,
                       rps_batch_done done, void *user, rps_stats *stats);
 
+/* Checks a request without touching a device: RPS_OK, or RPS_EINVAL with the
+ * reason rps_sieve would give written to `message` (NUL-terminated, at most
+ * message_len bytes; may be NULL).  No counterpart in the reference, whose
+ * checks run inside run_modular_sieve (src/sieve_plan.cpp:55-58); a server
+ * that batches many requests per launch uses it to refuse a bad request on
+ * admission instead of failing the whole launch. */
+int rps_check_request(const rps_request *request, char *message, size_t message_len);
+
 /* Survivor sink of a many-curve request: request[i] is the index into the
  * request array of survivor i.  Same validity, order and cancel rules as
  * rps_sink. */

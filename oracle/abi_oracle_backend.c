@@ -25,6 +25,8 @@ long long rpo_sieve_direct(const rpo_curve *curve, long long bound, int first,
                            int last, int square, int threads, long long *num,
                            int *den, long long cap, uint64_t digest[2]);
 
+int rps_check_request(const rps_request *req, char *message, size_t message_len);
+
 struct rps_ctx {
     char error[256];
     int batch_size;
@@ -124,9 +126,7 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
         return RPS_EINVAL;
     }
     ctx->error[0] = '\0';
-    if (req->den_first < 1 || req->den_first > req->den_last
-        || (req->square_denominators && req->den_last > 46340)) {
-        snprintf(ctx->error, sizeof ctx->error, "invalid denominator range");
+    if (rps_check_request(req, ctx->error, sizeof ctx->error) != RPS_OK) {
         return RPS_EINVAL;
     }
     double sites = (2.0 * (double)req->numerator_bound + 1.0)
@@ -199,4 +199,79 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
 int rps_sieve(rps_ctx *ctx, const rps_request *req, rps_sink sink, void *user,
               rps_stats *stats) {
     return rps_sieve_batched(ctx, req, sink, NULL, user, stats);
+}
+
+int rps_check_request(const rps_request *req, char *message, size_t message_len) {
+    const char *reason = "";
+    int rc = RPS_OK;
+    if (!req || req->n_coefficients < 2 || req->n_coefficients > 101 || !req->coefficients) {
+        reason = "supported polynomial degrees are 1 through 100";
+        rc = RPS_EINVAL;
+    } else if (req->numerator_bound < 1) {
+        reason = "invalid numerator bound";
+        rc = RPS_EINVAL;
+    } else if (req->den_first < 1 || req->den_first > req->den_last
+               || (req->square_denominators && req->den_last > 46340)) {
+        reason = "invalid denominator range";
+        rc = RPS_EINVAL;
+    }
+    if (message && message_len) {
+        snprintf(message, message_len, "%s", reason);
+    }
+    return rc;
+}
+
+/* Many requests: each searched alone, in reverse request order, survivors
+ * tagged with their request index. */
+typedef struct {
+    rps_multi_sink sink;
+    void *user;
+    int32_t request;
+    int32_t *tags;
+    size_t tags_len;
+} many_fwd;
+
+static int many_forward(void *u, const int64_t *num, const int32_t *den, size_t n) {
+    many_fwd *f = (many_fwd *)u;
+    if (n > f->tags_len) {
+        int32_t *grown = (int32_t *)realloc(f->tags, n * sizeof *grown);
+        if (!grown) {
+            return 1;
+        }
+        f->tags = grown;
+        f->tags_len = n;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        f->tags[i] = f->request;
+    }
+    return f->sink(f->user, f->tags, num, den, n);
+}
+
+int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
+                   rps_multi_sink sink, void *user, rps_stats *stats) {
+    rps_stats total;
+    memset(&total, 0, sizeof total);
+    if (!ctx || !sink || (!requests && n_requests)) {
+        return RPS_EINVAL;
+    }
+    for (size_t r = 0; r < n_requests; ++r) {
+        if (rps_check_request(&requests[r], ctx->error, sizeof ctx->error) != RPS_OK) {
+            return RPS_EINVAL;
+        }
+    }
+    many_fwd fwd = {sink, user, 0, NULL, 0};
+    int status = RPS_OK;
+    for (size_t k = n_requests; k-- > 0 && status == RPS_OK;) {
+        rps_stats one;
+        fwd.request = (int32_t)k;
+        status = rps_sieve(ctx, &requests[k], many_forward, &fwd, &one);
+        total.survivor_count += one.survivor_count;
+        total.launches += one.launches;
+    }
+    free(fwd.tags);
+    total.batches = n_requests;
+    if (stats) {
+        *stats = total;
+    }
+    return status;
 }

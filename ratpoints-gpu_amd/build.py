@@ -27,10 +27,11 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result",
 ]
 
-HOST_SOURCES = [
-    "command_line.cpp", "exact_polynomial.cpp", "output_formatter.cpp",
-    "point_search.cpp", "sieve_shim.cpp", "gpu_devices.cpp", "main.cpp",
+HOST_COMMON = [
+    "exact_polynomial.cpp", "point_search.cpp", "sieve_shim.cpp", "gpu_devices.cpp",
 ]
+HOST_SOURCES = HOST_COMMON + ["command_line.cpp", "output_formatter.cpp", "main.cpp"]
+SERVE_SOURCES = HOST_COMMON + ["serve.cpp", "serve_main.cpp"]
 
 
 def gmp_prefix():
@@ -69,17 +70,17 @@ def build_library():
     return out
 
 
-def build_cli(backend=None, out=None):
+def build_cli(backend=None, out=None, sources=None, name="ratpoints_gpu"):
     """The CLI over the HIP library, or -- for tests only -- over another
     implementation of the C ABI given as a list of object files."""
     host = os.path.join(HERE, "host")
-    sources = [os.path.join(host, s) for s in HOST_SOURCES]
+    sources = [os.path.join(host, s) for s in (sources or HOST_SOURCES)]
     if not all(os.path.exists(s) for s in sources):
         return None
     prefix = gmp_prefix()
     if prefix is None:
         raise RuntimeError("GMP (gmpxx.h) not found under /usr, /usr/local or /opt/conda")
-    out = out or os.path.join(HERE, "bin", "ratpoints_gpu")
+    out = out or os.path.join(HERE, "bin", name)
     os.makedirs(os.path.dirname(out), exist_ok=True)
     libdir = os.path.join(HERE, "lib")
     cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-Wextra", "-pthread",
@@ -113,6 +114,11 @@ def build_cli(backend=None, out=None):
     return out
 
 
+def build_serve(backend=None, out=None):
+    """bin/ratpoints_serve: the serving loop's trace driver."""
+    return build_cli(backend, out, SERVE_SOURCES, "ratpoints_serve")
+
+
 def build_probe():
     """bin/lds_roof: the LDS read-bandwidth probe behind the roofline peak."""
     os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
@@ -140,6 +146,7 @@ def build_oracle():
              "-c", os.path.join(oracle, src), "-o", obj])
         objs.append(obj)
     build_cli(backend=objs, out=os.path.join(build_dir, "ratpoints_gpu_oracle"))
+    build_serve(backend=objs, out=os.path.join(build_dir, "ratpoints_serve_oracle"))
 
 
 def build_all():
@@ -147,6 +154,7 @@ def build_all():
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     lib = build_library()
     cli = build_cli()
+    build_serve()
     build_probe()
     build_oracle()
     return lib, cli

@@ -376,14 +376,14 @@ void row_runs(rps::LineDesc &L, long long max_run) {
     L.tile_t = static_cast<int>(std::max(1LL, std::min(want, max_run)));
 }
 
-// Short lines: enough denominators per tile to give every wave of the
-// workgroup a full 64-pair slice, at most kDirectMaxDenominators.
-void direct_tiling(rps::LineDesc &L) {
-    const unsigned long long want = 64ULL * rps::kDirectWaves;
-    const int per = static_cast<int>(std::min<unsigned long long>(
-        rps::kDirectMaxDenominators, (want + L.words - 1) / L.words));
+// Short lines: a tile is a run of lines of about kDirectRunWords words in
+// all (swept in blocks of kDirectMaxDenominators lines), at most max_run lines.
+constexpr unsigned long long kDirectRunWords = 1ULL << 16;
+
+void direct_tiling(rps::LineDesc &L, long long max_run) {
+    const long long want = static_cast<long long>((kDirectRunWords + L.words - 1) / L.words);
     L.direct = 1;
-    L.tile_t = std::max(1, per);
+    L.tile_t = static_cast<int>(std::max(1LL, std::min(want, max_run)));
 }
 
 size_t tile_count(const rps::LineDesc &L) {
@@ -712,7 +712,7 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
             long long n = std::min<long long>(ctx->batch_size, last_t - next_t + 1);
             bt.line = base.line;
             if (words <= ctx->direct_words) {
-                direct_tiling(bt.line);
+                direct_tiling(bt.line, std::max(1LL, n / std::max(1LL, ctx->min_tiles)));
             } else {
                 unsigned long long chunks = line_chunks(ctx->min_tiles, n, words, stride,
                                                         var->waves);
@@ -823,6 +823,21 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
     }
 }
 
+int rps_check_request(const rps_request *request, char *message, size_t message_len) {
+    std::string reason;
+    int rc = RPS_OK;
+    try {
+        validate(request);
+    } catch (const ApiError &e) {
+        rc = e.code;
+        reason = e.message;
+    }
+    if (message && message_len) {
+        std::snprintf(message, message_len, "%s", reason.c_str());
+    }
+    return rc;
+}
+
 int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
                    rps_multi_sink sink, void *user, rps_stats *stats) {
     if (!ctx) {
@@ -867,7 +882,7 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
             L.first_t = requests[r].den_first;
             L.n_t = requests[r].den_last - requests[r].den_first + 1;
             if (L.words <= ctx->direct_words) {
-                direct_tiling(L);
+                direct_tiling(L, L.n_t);
             } else {
                 const unsigned long long chunks = std::max(
                     (L.words + (1ULL << 18) - 1) >> 18, 1ULL);

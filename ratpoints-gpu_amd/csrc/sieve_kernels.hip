@@ -43,6 +43,15 @@ __device__ __forceinline__ uint64_t window64(const uint64_t *c, uint32_t o) {
     return sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
 }
 
+// The same window from the string read as 32-bit words: two funnel shifts
+// of the three words from o / 32 (o + 63 < 576).
+__device__ __forceinline__ uint64_t window64_32(const uint32_t *c, uint32_t o) {
+    const uint32_t *d = c + (o >> 5);
+    const uint32_t lo = __builtin_amdgcn_alignbit(d[1], d[0], o & 31);
+    const uint32_t hi = __builtin_amdgcn_alignbit(d[2], d[1], o & 31);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
@@ -234,16 +243,28 @@ struct RowTile {
         const uint32_t *s_p = reinterpret_cast<const uint32_t *>(pool + kSmall);
         const uint32_t *s_c = s_p + kSelected;
         const uint32_t *s_magic = s_p + 5 * kSelected;
-        const uint64_t *cext = pool + kCext;
+        const uint32_t *cext = reinterpret_cast<const uint32_t *>(pool + kCext);
+        constexpr int kIters = (kSlot + kThreads - 1) / kThreads;
+        // Fully unrolled: every entry is independent, so all string reads of
+        // a thread can be in flight together.  The window at bit o is two
+        // funnel shifts of the three 32-bit string words from o / 32.
+#pragma unroll
         for (int s = 0; s < FAST; ++s) {
             const uint32_t p = s_p[s];
             const uint32_t c = s_c[s];
             const uint32_t magic = s_magic[s];
-            const uint64_t *cx = cext + s * 10;
-            const int n = static_cast<int>(p) + kStride - 1;
-            for (int e = tid; e < n; e += kThreads) {
-                const uint32_t o = mod_small(c + 64u * static_cast<uint32_t>(e), p, magic);
-                pool[s * kSlot + e] = window64(cx, o);
+            const uint32_t n = p + kStride - 1;
+            const uint32_t *cx = cext + s * 20;
+#pragma unroll
+            for (int k = 0; k < kIters; ++k) {
+                const uint32_t e = static_cast<uint32_t>(tid + k * kThreads);
+                if (e < n) {
+                    const uint32_t o = mod_small(c + 64u * e, p, magic);
+                    const uint32_t *d = cx + (o >> 5);
+                    const uint32_t lo = __builtin_amdgcn_alignbit(d[1], d[0], o & 31);
+                    const uint32_t hi = __builtin_amdgcn_alignbit(d[2], d[1], o & 31);
+                    pool[s * kSlot + e] = (static_cast<uint64_t>(hi) << 32) | lo;
+                }
             }
         }
     }
@@ -622,6 +643,37 @@ struct RowTileQ {
 };
 
 // ---------------------------------------------------------------------------
+// Candidate constants of a curve in LDS (u32 arrays p, magic, (-A) mod p of
+// 36 entries each) and the per-denominator prime selection from them.
+// ---------------------------------------------------------------------------
+
+// Threads 0..35 write the arrays; the caller syncs.
+__device__ __forceinline__ void load_candidates(uint32_t *cp, const LineDesc &L,
+                                                const int *primes, int tid) {
+    if (tid < kCandidates) {
+        const uint32_t p = static_cast<uint32_t>(primes[tid]);
+        cp[tid] = p;
+        cp[kCandidates + tid] = 0xffffffffu / p;  // floor(2^32 / p), p odd
+        cp[2 * kCandidates + tid] = static_cast<uint32_t>(L.neg_bound_mod[tid]);
+    }
+}
+
+// src/sieve_plan.cpp:75-86: the 32 largest candidates not dividing b.  Lane
+// i tests candidate i; its slot is the number of qualifying candidates above
+// it.  Returns the slot (or -1) with p, its magic and u = b mod p.
+__device__ __forceinline__ int select_rank(const uint32_t *cp, uint32_t b, int lane,
+                                           uint32_t &p, uint32_t &magic, uint32_t &u) {
+    const bool cand_lane = lane < kCandidates;
+    p = cand_lane ? cp[lane] : 1u;
+    magic = cand_lane ? cp[kCandidates + lane] : 0u;
+    u = cand_lane ? mod_small(b, p, magic) : 0u;
+    const bool ok = cand_lane && u != 0;
+    const unsigned long long mask = __ballot(ok);
+    const int rank = ok ? __popcll(mask >> (lane + 1)) : kSelected;
+    return rank < kSelected ? rank : -1;
+}
+
+// ---------------------------------------------------------------------------
 // Runs of lines.  A row tile covers nr consecutive denominators of one curve
 // (nr > 1 only when the tile is the whole line).  The curve's candidate
 // constants stay in LDS for the tile; per line, wave 0 selects the primes,
@@ -648,31 +700,9 @@ struct RowRun {
         return reinterpret_cast<uint32_t *>(pool + kCand);
     }
 
-    // Candidate constants of curve L (threads 0..35; the caller syncs).
     __device__ static void load_curve(uint64_t *pool, const LineDesc &L,
                                       const int *primes, int tid) {
-        uint32_t *cp = cand(pool);
-        if (tid < kCandidates) {
-            const uint32_t p = static_cast<uint32_t>(primes[tid]);
-            cp[tid] = p;
-            cp[kCandidates + tid] = 0xffffffffu / p;  // floor(2^32 / p), p odd
-            cp[2 * kCandidates + tid] = static_cast<uint32_t>(L.neg_bound_mod[tid]);
-        }
-    }
-
-    // src/sieve_plan.cpp:75-86: the 32 largest candidates not dividing b.
-    // Lane i tests candidate i; its slot is the number of qualifying
-    // candidates above it.  Returns the slot (or -1) and u = b mod p.
-    __device__ static int rank_of(const uint32_t *cp, uint32_t b, int lane,
-                                  uint32_t &p, uint32_t &magic, uint32_t &u) {
-        const bool cand_lane = lane < kCandidates;
-        p = cand_lane ? cp[lane] : 1u;
-        magic = cand_lane ? cp[kCandidates + lane] : 0u;
-        u = cand_lane ? mod_small(b, p, magic) : 0u;
-        const bool ok = cand_lane && u != 0;
-        const unsigned long long mask = __ballot(ok);
-        const int rank = ok ? __popcll(mask >> (lane + 1)) : kSelected;
-        return rank < kSelected ? rank : -1;
+        load_candidates(cand(pool), L, primes, tid);
     }
 
     // Wave 0: the selection of denominator b for the line chunk at ts into
@@ -681,7 +711,7 @@ struct RowRun {
         uint32_t *cp = cand(pool);
         uint32_t *s_p = reinterpret_cast<uint32_t *>(pool + Base::kSmall);
         uint32_t p, magic, u;
-        const int rank = rank_of(cp, b, lane, p, magic, u);
+        const int rank = select_rank(cp, b, lane, p, magic, u);
         if (rank >= 0) {
             s_p[rank] = p;
             s_p[kSelected + rank] = cp[2 * kCandidates + lane];  // c
@@ -697,7 +727,7 @@ struct RowRun {
         uint32_t *cp = cand(pool);
         uint32_t *nx = cp + 3 * kCandidates;
         uint32_t p, magic, u;
-        const int rank = rank_of(cp, b, lane, p, magic, u);
+        const int rank = select_rank(cp, b, lane, p, magic, u);
         if (rank >= 0) {
             nx[rank] = static_cast<uint32_t>(lane);
             nx[kSelected + rank] = u;
@@ -847,7 +877,7 @@ struct DirectTile {
                 const uint32_t pc = pp[2 * s];
                 const uint32_t p = pc & 0xffffu;
                 const uint32_t o = mod_small((pc >> 16) + 64u * w, p, pp[2 * s + 1]);
-                m[0] &= window64(cx + s * 10, o);
+                m[0] &= window64_32(reinterpret_cast<const uint32_t *>(cx + s * 10), o);
             }
             if (!__ballot(m[0] != 0)) {
                 continue;
@@ -858,6 +888,64 @@ struct DirectTile {
             }
         }
         return qe;
+    }
+};
+
+// Runs of short lines: a tile covers many lines of one curve, taken in blocks
+// of up to NB lines.  The curve's candidate constants stay in LDS for the
+// tile; per block, one wave per line selects the primes and the workgroup
+// copies the block's residue strings (a block's sweep is long next to that
+// one global round trip).
+template <int WAVES, int NB>
+struct DirectRun {
+    using Tile = DirectTile<WAVES, NB>;
+    static constexpr int kThreads = WAVES * 64;
+    static constexpr int kCand = Tile::kLdsWords;  // u32: p, magic, (-A) mod p [36 each]
+    static constexpr int kNext = kCand + (3 * kCandidates + 1) / 2;  // u32: index, u [NB][32 each]
+    static constexpr int kLdsWords = kNext + NB * kSelected;
+    static constexpr int kItems = NB * kSelected * kResidWords;
+
+    // Cursor of a wave: block index above bit 40, pair position below.
+    static constexpr int kBlockShift = 40;
+    static constexpr uint64_t kPosMask = (1ULL << kBlockShift) - 1;
+
+    __device__ static uint32_t *cand(uint64_t *pool) {
+        return reinterpret_cast<uint32_t *>(pool + kCand);
+    }
+
+    __device__ static void load_curve(uint64_t *pool, const LineDesc &L,
+                                      const int *primes, int tid) {
+        load_candidates(cand(pool), L, primes, tid);
+    }
+
+    // Set-up of block [t0, t0 + nb); every thread calls it after the
+    // previous block's sweep.  Ends with a barrier.
+    __device__ static void setup(uint64_t *pool, const LineDesc &L, int t0, int nb,
+                                 int tid, int lane, int wave) {
+        const uint32_t *cp = cand(pool);
+        uint32_t *prm = reinterpret_cast<uint32_t *>(pool + Tile::kPrm);
+        uint32_t *nx = reinterpret_cast<uint32_t *>(pool + kNext);
+        for (int bl = wave; bl < nb; bl += WAVES) {
+            const uint32_t b = static_cast<uint32_t>(
+                denominator(L, static_cast<long long>(t0) + bl));
+            uint32_t p, magic, u;
+            const int rank = select_rank(cp, b, lane, p, magic, u);
+            if (rank >= 0) {
+                const int slot = bl * kSelected + rank;
+                prm[2 * slot] = p | (cp[2 * kCandidates + lane] << 16);
+                prm[2 * slot + 1] = magic;
+                nx[slot] = static_cast<uint32_t>(lane);
+                nx[NB * kSelected + slot] = u;
+            }
+        }
+        __syncthreads();
+        for (int j = tid; j < nb * kSelected * kResidWords; j += kThreads) {
+            const int slot = j / kResidWords;
+            const int w = j - slot * kResidWords;
+            pool[slot * 10 + w] = L.resid[(static_cast<size_t>(nx[slot]) * kMaxPrime
+                                           + nx[NB * kSelected + slot]) * kResidWords + w];
+        }
+        __syncthreads();
     }
 };
 
@@ -1035,40 +1123,67 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4)))
     }
 }
 
+// Tile = run of L.tile_t short lines, swept in blocks of NB lines; each wave
+// keeps a (block, pair) cursor.
 template <int WAVES, int NB>
 __global__ __launch_bounds__(WAVES * 64) void direct_kernel(const SieveParams P) {
+    using Run = DirectRun<WAVES, NB>;
     using Tile = DirectTile<WAVES, NB>;
-    __shared__ uint64_t pool[Tile::kLdsWords];
-    __shared__ int s_live;
+    __shared__ uint64_t pool[Run::kLdsWords];
+    __shared__ unsigned s_first;
     const LineDesc &L = P.line;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = static_cast<int>(uniform(tid >> 6));
     const int tile = blockIdx.x;
     const int t0 = L.first_t + tile * L.tile_t;
-    const int nb = L.n_t - tile * L.tile_t < L.tile_t ? L.n_t - tile * L.tile_t : L.tile_t;
-    const uint32_t total = static_cast<uint32_t>(nb) * static_cast<uint32_t>(L.words);
-    uint32_t qs, qe;
-    Tile::wave_range(total, wave, qs, qe);
+    const int rest = L.n_t - tile * L.tile_t;
+    const int nr = rest < L.tile_t ? rest : L.tile_t;
+    const int nblocks = (nr + NB - 1) / NB;
+    const uint32_t words = static_cast<uint32_t>(L.words);
     unsigned long long *cursor = P.cursors + static_cast<size_t>(tile) * WAVES + wave;
+    uint32_t ck = 0, cq = 0;
     if (P.resume) {
-        qs = static_cast<uint32_t>(*cursor);
+        const uint64_t c = uniform64(*cursor);
+        ck = static_cast<uint32_t>(c >> Run::kBlockShift);
+        cq = static_cast<uint32_t>(c & Run::kPosMask);
     }
     if (tid == 0) {
-        s_live = 0;
+        s_first = static_cast<unsigned>(nblocks);
+    }
+    Run::load_curve(pool, L, P.primes, tid);
+    __syncthreads();
+    if (lane == 0 && ck < static_cast<uint32_t>(nblocks)) {
+        atomicMin(&s_first, ck);
     }
     __syncthreads();
-    if (lane == 0 && qs < qe) {
-        atomicOr(&s_live, 1);
-    }
-    __syncthreads();
-    if (!s_live) {
+    const int k0 = static_cast<int>(uniform(s_first));
+    if (k0 >= nblocks) {
         return;
     }
-    Tile::setup(pool, t0, nb, L, P.primes, lane, wave);
-    const uint32_t stop = qs < qe ? Tile::sweep(pool, t0, qs, qe, L, P.out, 0, lane) : qe;
+    bool failed = false;
+    for (int k = k0; k < nblocks; ++k) {
+        const int tk = t0 + k * NB;
+        const int nb = nr - k * NB < NB ? nr - k * NB : NB;
+        __syncthreads();  // the previous block's sweep is over
+        Run::setup(pool, L, tk, nb, tid, lane, wave);
+        uint32_t qs, qe;
+        Tile::wave_range(static_cast<uint32_t>(nb) * words, wave, qs, qe);
+        if (!failed && static_cast<uint32_t>(k) >= ck && qs < qe) {
+            const uint32_t from = P.resume && static_cast<uint32_t>(k) == ck ? cq : qs;
+            if (from < qe) {
+                const uint32_t stop = Tile::sweep(pool, tk, from, qe, L, P.out, 0, lane);
+                if (stop != qe) {
+                    failed = true;
+                    ck = static_cast<uint32_t>(k);
+                    cq = stop;
+                }
+            }
+        }
+    }
     if (lane == 0) {
-        *cursor = stop;
+        *cursor = failed ? (static_cast<uint64_t>(ck) << Run::kBlockShift) | cq
+                         : static_cast<uint64_t>(nblocks) << Run::kBlockShift;
     }
 }
 
@@ -1088,8 +1203,9 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4)))
     const MultiParams M) {
     using Run = RowRun<Row>;
     using Direct = DirectTile<WAVES, kDirectMaxDenominators>;
-    constexpr int kWords = Run::kLdsWords > Direct::kLdsWords ? Run::kLdsWords
-                                                               : Direct::kLdsWords;
+    using DRun = DirectRun<WAVES, kDirectMaxDenominators>;
+    constexpr int kWords = Run::kLdsWords > DRun::kLdsWords ? Run::kLdsWords
+                                                             : DRun::kLdsWords;
     __shared__ uint64_t pool[kWords];
     __shared__ unsigned long long s_tile;
     __shared__ int s_curve;
@@ -1141,15 +1257,32 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4)))
             const unsigned long long local = tile - C.tile_start;
             bool failed = false;
             if (L.direct) {
+                constexpr int NB = kDirectMaxDenominators;
                 const int t0 = L.first_t + static_cast<int>(local) * L.tile_t;
                 const int rest = L.n_t - static_cast<int>(local) * L.tile_t;
-                const int nb = rest < L.tile_t ? rest : L.tile_t;
-                uint32_t qs, qe;
-                Direct::wave_range(static_cast<uint32_t>(nb) * static_cast<uint32_t>(L.words),
-                                   wave, qs, qe);
-                Direct::setup(pool, t0, nb, L, M.primes, lane, wave);
-                if (qs < qe) {
-                    failed = Direct::sweep(pool, t0, qs, qe, L, M.out, C.request, lane) != qe;
+                const int nr = rest < L.tile_t ? rest : L.tile_t;
+                const int nblocks = (nr + NB - 1) / NB;
+                DRun::load_curve(pool, L, M.primes, tid);
+                for (int k = 0; k < nblocks; ++k) {
+                    const int tk = t0 + k * NB;
+                    const int nb = nr - k * NB < NB ? nr - k * NB : NB;
+                    __syncthreads();  // the previous block's sweep is over
+                    // s_fail is only written during sweeps, so every wave
+                    // reads the same value here.
+                    if (uniform(static_cast<uint32_t>(s_fail))) {
+                        break;
+                    }
+                    DRun::setup(pool, L, tk, nb, tid, lane, wave);
+                    uint32_t qs, qe;
+                    Direct::wave_range(static_cast<uint32_t>(nb) * static_cast<uint32_t>(L.words),
+                                       wave, qs, qe);
+                    if (qs < qe
+                        && Direct::sweep(pool, tk, qs, qe, L, M.out, C.request, lane) != qe) {
+                        failed = true;
+                        if (lane == 0) {
+                            s_fail = 1;
+                        }
+                    }
                 }
             } else {
                 const int run = static_cast<int>(local / static_cast<unsigned>(L.tiles_per_t));
@@ -1227,9 +1360,7 @@ const VariantInfo kVariants[] = {
     RPS_VARIANT_Q(2, 10, 8),
     RPS_VARIANT_Q(1, 12, 8),
     RPS_VARIANT_Q(2, 12, 16),
-    RPS_VARIANT_Q(4, 12, 16),
     RPS_VARIANT(2, 14, 8),
-    RPS_VARIANT(2, 14, 16),
 };
 
 #undef RPS_VARIANT

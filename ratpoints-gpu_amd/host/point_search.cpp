@@ -66,8 +66,10 @@ void verify_range(const CandidateBatch &batch, size_t begin, size_t end,
     }
 }
 
-void verify(const CandidateBatch &batch, const ExactPolynomial &polynomial,
-            std::vector<PointPair> &out) {
+}  // namespace
+
+void verify_candidates(const CandidateBatch &batch, const ExactPolynomial &polynomial,
+                       std::vector<PointPair> &out) {
     const size_t n = batch.size();
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     const size_t workers = std::min<size_t>(std::min(hw, 16u), n / 4096);
@@ -102,6 +104,8 @@ void order_points(std::vector<PointPair> &points) {
                       : x.numerator < y.numerator;
               });
 }
+
+namespace {
 
 void add_count(unsigned long long &total, unsigned long long value,
                const char *what) {
@@ -179,7 +183,7 @@ SearchMetrics PointSearch::run(const PointCallback &point_callback) {
     bool stopped = false;
 
     CandidateBatchCallback on_survivors = [&](const CandidateBatch &batch) {
-        verify(batch, impl_->polynomial, pending);
+        verify_candidates(batch, impl_->polynomial, pending);
     };
     BatchDoneCallback on_batch = [&](DenominatorRange range,
                                      std::uint64_t survivors) {

@@ -71,6 +71,18 @@ struct SearchMetrics {
 // Receives points in output order; return false to stop the search.
 using PointCallback = std::function<bool(const PointPair &)>;
 
+class ExactPolynomial;
+struct CandidateBatch;
+
+// Exact check of modular survivors (src/point_search.cpp:48-60): appends
+// (a, sqrt F(a, b), b) for every survivor with gcd(a, b) = 1 and F(a, b) a
+// perfect square; large batches are split over host threads.
+void verify_candidates(const CandidateBatch &batch, const ExactPolynomial &polynomial,
+                       std::vector<PointPair> &out);
+
+// Output order: increasing (b, a).
+void order_points(std::vector<PointPair> &points);
+
 class PointSearch {
 public:
     explicit PointSearch(SearchOptions options);

@@ -231,3 +231,55 @@ def config3_curves(seed=3, count=4096):
         out.append({"coefficients": coeffs, "bound": bound, "first": 1, "last": den,
                     "points": points, "degree": degree})
     return out
+
+
+def _unit(rng):
+    return (rng.next() >> 11) / float(1 << 53)
+
+
+def config5_requests(seed=5, count=64, qps=2.0, min_height=100, max_height=10_000_000):
+    """Serving mix: Poisson arrivals at `qps` requests/s; degree uniform in
+    2..12, coefficient size uniform in 16..512 bits, height H log-uniform in
+    [min_height, max_height] (|a| <= H, 1 <= b <= H), 0-2 planted points
+    (the batched construction, so first-point requests have points to stop
+    at), 30% in -1 first-point mode, 10% with 1-3 closed x-intervals.
+    Returns dicts {id, arrival_ms, coefficients, bound, first, last,
+    first_point, intervals [(lo, hi) Fractions, None = infinite], points}."""
+    rng = SplitMix64(seed)
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += -math.log(1.0 - _unit(rng)) / qps * 1000.0
+        degree = rng.uniform(2, 12)
+        bits = rng.uniform(16, 512)
+        height = _log_uniform_int(rng, min_height, max_height)
+        n_points = rng.uniform(0, 2)
+        coeffs, points = batched_curve(rng, degree, bits, n_points, height, height)
+        first_point = _unit(rng) < 0.3
+        intervals = []
+        if _unit(rng) < 0.1:
+            for _ in range(rng.uniform(1, 3)):
+                lo = Fraction(rng.uniform(-40, 40), rng.uniform(1, 8))
+                hi = lo + Fraction(rng.uniform(1, 60), rng.uniform(1, 8))
+                intervals.append((None if _unit(rng) < 0.2 else lo,
+                                  None if _unit(rng) < 0.2 else hi))
+        out.append({"id": i, "arrival_ms": t, "coefficients": coeffs, "bound": height,
+                    "first": 1, "last": height, "first_point": first_point,
+                    "intervals": intervals, "points": points, "degree": degree})
+    return out
+
+
+def write_trace(path, requests, square=False, no_infinity=False):
+    """Writes requests in the trace format of bin/ratpoints_serve."""
+    with open(path, "w") as fh:
+        for r in requests:
+            flags = (1 if r.get("first_point") else 0) | (2 if r.get("square", square) else 0) \
+                | (4 if r.get("no_infinity", no_infinity) else 0)
+            ivs = r.get("intervals", [])
+            parts = [str(r["id"]), "%.3f" % r["arrival_ms"], str(flags), str(r["bound"]),
+                     str(r["first"]), str(r["last"]), str(len(ivs))]
+            for lo, hi in ivs:
+                parts += ["-inf" if lo is None else str(lo), "inf" if hi is None else str(hi)]
+            parts.append(str(len(r["coefficients"])))
+            parts += [str(c) for c in r["coefficients"]]
+            fh.write(" ".join(parts) + "\n")

@@ -183,6 +183,32 @@ def test_runs_resume_exactly_once(fresh_gpu, oracle_lib, pkg, variant):
     assert_same(res, oracle_lib.sieve(coeffs, 40000, 300, 450, method="words"))
 
 
+@pytest.mark.parametrize("bound", [3000, 20000])
+def test_short_line_runs(fresh_gpu, oracle_lib, bound):
+    # min_tiles = 1: a short-line tile takes a run of up to 2^16 words of
+    # lines, swept in blocks of 16 (94-word lines: 300 lines in one tile, 19
+    # blocks; 626-word lines: 105 lines per tile).
+    coeffs, _ = curves.partially_dense(random.Random(140 + bound), 7, 6)
+    announce(case="short-runs", A=bound)
+    fresh_gpu.set_option("min_tiles", 1)
+    res = fresh_gpu.sieve(coeffs, bound, 5, 304)
+    assert_same(res, oracle_lib.sieve(coeffs, bound, 5, 304, method="words"))
+    res = fresh_gpu.sieve(coeffs, bound, 1, 150, square=True)
+    assert_same(res, oracle_lib.sieve(coeffs, bound, 1, 150, square=True, method="words"))
+
+
+def test_short_line_runs_resume(fresh_gpu, oracle_lib):
+    # Dense survivors in runs of short lines: waves stop mid-run on a full
+    # 16,384-survivor buffer and resume at their (block, pair) cursors.
+    coeffs, _ = curves.partially_dense(random.Random(150), 2, 6)
+    announce(case="short-runs-resume", capacity=16384)
+    fresh_gpu.set_option("min_tiles", 1)
+    fresh_gpu.set_option("survivor_capacity", 16384)
+    res = fresh_gpu.sieve(coeffs, 5000, 2, 401)
+    assert res.stats["launches"] >= res.survivor_count // 16384
+    assert_same(res, oracle_lib.sieve(coeffs, 5000, 2, 401, method="words"))
+
+
 def test_line_split_into_chunks(fresh_gpu, oracle_lib):
     # Few denominators and a long line: each line is cut into many tiles.
     coeffs, _ = curves.partially_dense(random.Random(8), 9, 4)

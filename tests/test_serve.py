@@ -1,0 +1,155 @@
+"""The serving loop (bin/ratpoints_serve): requests that arrive while others
+run are admitted and sieved together in rounds; every request's result must
+equal its search alone, and both must equal the exact point set.
+
+Bar: per request, the emitted point list (infinity first, then (b, a)
+order, interval filter, -1 truncation) is identical between --mode serve and
+--mode alone, and identical to the oracle's brute-force exact points for the
+small boxes used here; requests that cannot be searched fail alone and in the
+mix.  The CPU tests drive the host logic over the oracle's CPU test double of
+the C ABI; the GPU test drives the HIP library.
+"""
+
+import os
+import subprocess
+from fractions import Fraction
+
+import pytest
+
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DOUBLE = os.path.join(ROOT, "oracle", "_build", "ratpoints_serve_oracle")
+
+
+def parse_results(path):
+    out = {}
+    with open(path) as fh:
+        for line in fh:
+            parts = line.split()
+            rid = int(parts[0])
+            if parts[1] == "error":
+                out[rid] = ("error", " ".join(parts[2:]))
+                continue
+            n = int(parts[4])
+            pts = []
+            for item in parts[5:5 + n]:
+                a, y, b = item.split(":")
+                pts.append((int(a), int(y), int(b)))
+            out[rid] = ("ok", pts)
+    return out
+
+
+def expected_points(r):
+    """Infinity first (unless suppressed), then exact points in (b, a) order
+    inside the intervals, cut to one for -1."""
+    pts = []
+    if not r.get("no_infinity"):
+        inf = oracle.infinity_point(r["coefficients"])
+        if inf is not None:
+            pts.append(inf)
+    ivs = r.get("intervals", [])
+
+    def accepts(a, b):
+        x = Fraction(a, b)
+        return not ivs or any((lo is None or x >= lo) and (hi is None or x <= hi)
+                              for lo, hi in ivs)
+
+    for a, y, b in oracle.exact_points(r["coefficients"], r["bound"], r["first"], r["last"],
+                                       r.get("square", False)):
+        if accepts(a, b):
+            pts.append((a, y, b))
+    return pts[:1] if r.get("first_point") else pts
+
+
+def small_trace(workloads, seed, count, max_height, qps):
+    reqs = workloads.config5_requests(seed=seed, count=count, qps=qps, min_height=20,
+                                      max_height=max_height)
+    t = reqs[-1]["arrival_ms"]
+    extra = [
+        # square denominators, no point at infinity
+        {"coefficients": [1, -1, 0, 1], "bound": 40, "first": 1, "last": 30, "square": True,
+         "no_infinity": True},
+        # planted sextic with -1 and an interval that excludes some points
+        {"coefficients": [4, 0, -3, 0, 0, 0, 1], "bound": 60, "first": 1, "last": 60,
+         "first_point": True, "intervals": [(Fraction(0), None)]},
+        # empty denominator range: only the point at infinity
+        {"coefficients": [1, 2, 3, 4, 9], "bound": 10, "first": 5, "last": 4},
+        # not squarefree: fails alone and in the mix
+        {"coefficients": [0, 0, 1], "bound": 10, "first": 1, "last": 10, "error": True},
+        # invalid numerator bound
+        {"coefficients": [1, 1, 1], "bound": 0, "first": 1, "last": 10, "error": True},
+    ]
+    for k, e in enumerate(extra):
+        e.setdefault("intervals", [])
+        e["id"] = count + k
+        e["arrival_ms"] = t * (k + 1) / len(extra)
+        reqs.append(e)
+    return reqs
+
+
+def run_modes(binary, trace, tmp_path, extra_args=()):
+    results = {}
+    for mode in ("serve", "alone"):
+        out = tmp_path / f"{mode}.txt"
+        proc = subprocess.run([binary, "--trace", str(trace), "--mode", mode, "--out", str(out),
+                               *extra_args], capture_output=True, text=True, timeout=600)
+        assert proc.returncode == 0, proc.stderr
+        print(mode, proc.stdout.strip(), flush=True)
+        results[mode] = parse_results(out)
+    return results
+
+
+def check(reqs, results, oracle_limit):
+    serve, alone = results["serve"], results["alone"]
+    assert set(serve) == set(alone) == {r["id"] for r in reqs}
+    for r in reqs:
+        s, a = serve[r["id"]], alone[r["id"]]
+        assert s[0] == a[0], (r["id"], s, a)
+        if r.get("error"):
+            assert s[0] == "error"
+            continue
+        assert s[0] == "ok", (r["id"], s)
+        assert s[1] == a[1], r["id"]
+        if (2 * r["bound"] + 1) * max(0, r["last"] - r["first"] + 1) <= oracle_limit:
+            assert s[1] == expected_points(r), r["id"]
+        for p in r.get("points", []):
+            if not r.get("first_point") and not r.get("intervals"):
+                a_, b_ = p
+                assert any(x[0] == a_ and x[2] == b_ for x in s[1]), (r["id"], p)
+
+
+def test_serve_matches_alone_on_double(workloads, tmp_path):
+    if not os.path.exists(DOUBLE):
+        pytest.skip("oracle serve double not built")
+    reqs = small_trace(workloads, 51, 12, 90, qps=400.0)
+    trace = tmp_path / "trace.txt"
+    workloads.write_trace(str(trace), reqs)
+    # Small rounds and few slots: requests queue for admission and take
+    # several rounds each.
+    results = run_modes(DOUBLE, trace, tmp_path,
+                        ("--round-sites", "3000", "--max-active", "4"))
+    check(reqs, results, oracle_limit=10 ** 9)
+
+
+def test_serve_trace_parse_errors(tmp_path):
+    if not os.path.exists(DOUBLE):
+        pytest.skip("oracle serve double not built")
+    bad = tmp_path / "bad.txt"
+    bad.write_text("0 0 0 10 1 5 0 3 1 x\n")
+    proc = subprocess.run([DOUBLE, "--trace", str(bad)], capture_output=True, text=True)
+    assert proc.returncode == 2
+    proc = subprocess.run([DOUBLE, "--mode", "fast"], capture_output=True, text=True)
+    assert proc.returncode == 2 and "usage" in proc.stderr
+
+
+@pytest.mark.gpu
+def test_serve_matches_alone_on_gpu(pkg, workloads, tmp_path):
+    binary = os.path.join(os.path.dirname(pkg.LIB_PATH), "..", "bin", "ratpoints_serve")
+    assert os.path.exists(binary), "bin/ratpoints_serve not built"
+    reqs = small_trace(workloads, 52, 40, 20000, qps=200.0)
+    trace = tmp_path / "trace.txt"
+    workloads.write_trace(str(trace), reqs)
+    print("case: serve-gpu requests=%d" % len(reqs), flush=True)
+    results = run_modes(binary, trace, tmp_path, ("--round-sites", "2e8", "--max-active", "16"))
+    check(reqs, results, oracle_limit=150_000)
