@@ -29,10 +29,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 BATCH = 65536
 BOUND = 5_000_000
 LAST = 5_000_000
-# LDS bytes the dense stage reads per site: 14 primes x one 64-bit row word
-# per 64 numerators (the reference's fast stage reads 14 x 32-bit words per
-# 32 numerators, the same 1.75 B/site).
-LDS_BYTES_PER_SITE = 14 * 8 / 64
 # MI355X LDS read bandwidth: 256 CUs x 128 B/clk x 2.4 GHz.
 LDS_PEAK_GBS = 256 * 128 * 2.4
 RTX2000_ADA_GSITES = 879.5   # published, A = B = 5e6 (README.md:40-47)
@@ -146,6 +142,59 @@ def batched_leg(pkg, wl):
     }
 
 
+def dense_primes(variant):
+    """Dense primes of a kernel variant from its name ("q_u2_f12_w8": 12)."""
+    for part in variant.split("_"):
+        if part.startswith("f") and part[1:].isdigit():
+            return int(part[1:])
+    raise ValueError(variant)
+
+
+def serving_leg(wl, count, qps):
+    """Config 5: Poisson arrivals of `count` requests at `qps` over the
+    seeded serving mix, through bin/ratpoints_serve (continuous batching)
+    and through the one-request-at-a-time loop on the same GPU; per-request
+    point lists must be identical."""
+    import subprocess
+    import tempfile
+    binary = os.path.join(ROOT, "ratpoints-gpu_amd", "bin", "ratpoints_serve")
+    reqs = wl.config5_requests(seed=5, count=count, qps=qps)
+    sites = sum((2 * r["bound"] + 1) * (r["last"] - r["first"] + 1) for r in reqs)
+    work = tempfile.mkdtemp(prefix="rps_serve_")
+    trace = os.path.join(work, "trace.txt")
+    wl.write_trace(trace, reqs)
+    summaries, points = {}, {}
+    for mode in ("serve", "alone"):
+        print(f"bench: serving {mode} ({count} requests, qps {qps}, box {sites:.3e} sites)",
+              file=sys.stderr, flush=True)
+        out = os.path.join(work, mode + ".txt")
+        proc = subprocess.run([binary, "--trace", trace, "--mode", mode, "--out", out],
+                              capture_output=True, text=True, timeout=1200)
+        if proc.returncode != 0:
+            raise RuntimeError(f"ratpoints_serve --mode {mode} failed: {proc.stderr[-2000:]}")
+        summaries[mode] = json.loads(proc.stdout.strip().splitlines()[-1])
+        with open(out) as fh:
+            points[mode] = {line.split()[0]: (line.split()[1:2], line.split()[4:])
+                            for line in fh}
+    same = all(points["serve"][k][1] == points["alone"][k][1] for k in points["alone"])
+
+    def pick(d):
+        return {k: d[k] for k in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "makespan_s",
+                                  "sites", "gsites_per_s", "rounds", "errors", "points")}
+
+    return {
+        "workload": "config 5: %d requests, Poisson %.1f/s, seed 5 (degrees 2-12, "
+                    "16-512-bit coefficients, height log-uniform 1e2..1e7, 30%% -1, "
+                    "10%% intervals)" % (count, qps),
+        "box_sites": sites,
+        "serve": pick(summaries["serve"]) | {"peak_active": summaries["serve"]["peak_active"]},
+        "alone": pick(summaries["alone"]),
+        "p50_speedup": round(summaries["alone"]["p50_ms"] / max(summaries["serve"]["p50_ms"], 1e-9), 2),
+        "p99_speedup": round(summaries["alone"]["p99_ms"] / max(summaries["serve"]["p99_ms"], 1e-9), 2),
+        "identical_to_alone": same,
+    }
+
+
 def verify_points(coeffs, nums, dens):
     """Exact stage on the survivors (gcd = 1, F(a, b) a perfect square)."""
     n = len(coeffs) - 1
@@ -171,6 +220,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched (4,096-curve) leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-qps", type=float, default=2.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; one MI355X covers the whole "
@@ -221,7 +273,10 @@ def main():
     gsites = sites / wall / 1e9
     kernel_gsites = kernel_sites / (kernel_ms / 1e3) / 1e9 if kernel_ms else 0.0
     per_launch_ms = kernel_ms / launches if launches else 0.0
-    lds_bytes_per_launch = LDS_BYTES_PER_SITE * kernel_sites / max(launches, 1)
+    # LDS bytes the dense stage reads per site: one 64-bit row word per dense
+    # prime per 64 numerators.
+    lds_bytes_per_site = dense_primes(variant) * 8 / 64
+    lds_bytes_per_launch = lds_bytes_per_site * kernel_sites / max(launches, 1)
     achieved = lds_bytes_per_launch / (per_launch_ms / 1e3) / 1e9 if per_launch_ms else 0.0
 
     result = {
@@ -254,7 +309,7 @@ def main():
             "traffic": measured_traffic(variant),
             "kernel": variant,
             "kernel_ms_per_launch": round(per_launch_ms, 3),
-            "bytes_per_site": LDS_BYTES_PER_SITE,
+            "bytes_per_site": lds_bytes_per_site,
         },
         "kernel_gsites": round(kernel_gsites, 2),
         "survivors": int(stats["survivor_count"]),
@@ -270,6 +325,10 @@ def main():
         result["batched"] = batched_leg(pkg, wl)
         if not result["batched"]["identical_to_plain_loop"]:
             result["error"] = "batched survivors differ from the plain loop"
+    if not args.no_serve:
+        result["serving"] = serving_leg(wl, args.serve_requests, args.serve_qps)
+        if not result["serving"]["identical_to_alone"]:
+            result["error"] = "served results differ from the searches alone"
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(coeffs)
