@@ -244,28 +244,56 @@ struct RowTile {
         const uint32_t *s_c = s_p + kSelected;
         const uint32_t *s_magic = s_p + 5 * kSelected;
         const uint32_t *cext = reinterpret_cast<const uint32_t *>(pool + kCext);
-        constexpr int kIters = (kSlot + kThreads - 1) / kThreads;
-        // Fully unrolled: every entry is independent, so all string reads of
-        // a thread can be in flight together.  The window at bit o is two
-        // funnel shifts of the three 32-bit string words from o / 32.
+        // Row constants as wave-uniform (scalar) values.
+        uint32_t P[FAST], C[FAST], M[FAST];
 #pragma unroll
         for (int s = 0; s < FAST; ++s) {
-            const uint32_t p = s_p[s];
-            const uint32_t c = s_c[s];
-            const uint32_t magic = s_magic[s];
-            const uint32_t n = p + kStride - 1;
-            const uint32_t *cx = cext + s * 20;
+            P[s] = uniform(s_p[s]);
+            C[s] = uniform(s_c[s]);
+            M[s] = uniform(s_magic[s]);
+        }
+        // Branch-free: thread t writes entry t of every row, and entry
+        // t + kThreads while that is inside the row's slot (a wave-uniform
+        // test, as kSlot and kThreads are multiples of 64).  Entries from
+        // p + kStride - 1 to the end of the slot are padding the sweep never
+        // reads, so they are written without a test.  All string reads of a
+        // thread are independent and can be in flight together; the window
+        // at bit o is two funnel shifts of three 32-bit string words.
+        static_assert(kSlot % 64 == 0 && kThreads % 64 == 0, "wave-uniform slot tests");
+        // Offsets first, then every string read, then the funnel shifts and
+        // the row stores, so a thread's reads are all in flight together and
+        // none waits behind a store to the same LDS array.
+        const uint32_t e0 = static_cast<uint32_t>(tid);
+        if (e0 < static_cast<uint32_t>(kSlot)) {
+            build_entries(pool, cext, P, C, M, e0);
+        }
+        if (kThreads < kSlot && e0 < static_cast<uint32_t>(kSlot - kThreads)) {
+            build_entries(pool, cext, P, C, M, e0 + kThreads);
+        }
+    }
+
+    // Entry e of every dense row.
+    __device__ static void build_entries(uint64_t *pool, const uint32_t *cext,
+                                         const uint32_t (&P)[FAST], const uint32_t (&C)[FAST],
+                                         const uint32_t (&M)[FAST], uint32_t e) {
+        uint32_t o[FAST];
 #pragma unroll
-            for (int k = 0; k < kIters; ++k) {
-                const uint32_t e = static_cast<uint32_t>(tid + k * kThreads);
-                if (e < n) {
-                    const uint32_t o = mod_small(c + 64u * e, p, magic);
-                    const uint32_t *d = cx + (o >> 5);
-                    const uint32_t lo = __builtin_amdgcn_alignbit(d[1], d[0], o & 31);
-                    const uint32_t hi = __builtin_amdgcn_alignbit(d[2], d[1], o & 31);
-                    pool[s * kSlot + e] = (static_cast<uint64_t>(hi) << 32) | lo;
-                }
-            }
+        for (int s = 0; s < FAST; ++s) {
+            o[s] = mod_small(C[s] + 64u * e, P[s], M[s]);
+        }
+        uint32_t d0[FAST], d1[FAST], d2[FAST];
+#pragma unroll
+        for (int s = 0; s < FAST; ++s) {
+            const uint32_t *d = cext + s * 20 + (o[s] >> 5);
+            d0[s] = d[0];
+            d1[s] = d[1];
+            d2[s] = d[2];
+        }
+#pragma unroll
+        for (int s = 0; s < FAST; ++s) {
+            const uint32_t lo = __builtin_amdgcn_alignbit(d1[s], d0[s], o[s] & 31);
+            const uint32_t hi = __builtin_amdgcn_alignbit(d2[s], d1[s], o[s] & 31);
+            pool[s * kSlot + e] = (static_cast<uint64_t>(hi) << 32) | lo;
         }
     }
 
