@@ -155,28 +155,9 @@ def serving_leg(wl, count, qps):
     seeded serving mix, through bin/ratpoints_serve (continuous batching)
     and through the one-request-at-a-time loop on the same GPU; per-request
     point lists must be identical."""
-    import subprocess
-    import tempfile
-    binary = os.path.join(ROOT, "ratpoints-gpu_amd", "bin", "ratpoints_serve")
     reqs = wl.config5_requests(seed=5, count=count, qps=qps)
     sites = sum((2 * r["bound"] + 1) * (r["last"] - r["first"] + 1) for r in reqs)
-    work = tempfile.mkdtemp(prefix="rps_serve_")
-    trace = os.path.join(work, "trace.txt")
-    wl.write_trace(trace, reqs)
-    summaries, points = {}, {}
-    for mode in ("serve", "alone"):
-        print(f"bench: serving {mode} ({count} requests, qps {qps}, box {sites:.3e} sites)",
-              file=sys.stderr, flush=True)
-        out = os.path.join(work, mode + ".txt")
-        proc = subprocess.run([binary, "--trace", trace, "--mode", mode, "--out", out],
-                              capture_output=True, text=True, timeout=1200)
-        if proc.returncode != 0:
-            raise RuntimeError(f"ratpoints_serve --mode {mode} failed: {proc.stderr[-2000:]}")
-        summaries[mode] = json.loads(proc.stdout.strip().splitlines()[-1])
-        with open(out) as fh:
-            points[mode] = {line.split()[0]: (line.split()[1:2], line.split()[4:])
-                            for line in fh}
-    same = all(points["serve"][k][1] == points["alone"][k][1] for k in points["alone"])
+    summaries, same = run_serve_modes(wl, reqs, "serving")
 
     def pick(d):
         return {k: d[k] for k in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "makespan_s",
@@ -192,6 +173,56 @@ def serving_leg(wl, count, qps):
         "p50_speedup": round(summaries["alone"]["p50_ms"] / max(summaries["serve"]["p50_ms"], 1e-9), 2),
         "p99_speedup": round(summaries["alone"]["p99_ms"] / max(summaries["serve"]["p99_ms"], 1e-9), 2),
         "identical_to_alone": same,
+    }
+
+
+def run_serve_modes(wl, reqs, label, extra=()):
+    """Both modes of bin/ratpoints_serve on one trace: summaries and
+    whether every request's point list is the same."""
+    import subprocess
+    import tempfile
+    binary = os.path.join(ROOT, "ratpoints-gpu_amd", "bin", "ratpoints_serve")
+    work = tempfile.mkdtemp(prefix="rps_serve_")
+    trace = os.path.join(work, "trace.txt")
+    wl.write_trace(trace, reqs)
+    summaries, points = {}, {}
+    for mode in ("serve", "alone"):
+        print(f"bench: {label} {mode} ({len(reqs)} requests)", file=sys.stderr, flush=True)
+        out = os.path.join(work, mode + ".txt")
+        proc = subprocess.run([binary, "--trace", trace, "--mode", mode, "--out", out, *extra],
+                              capture_output=True, text=True, timeout=1200)
+        if proc.returncode != 0:
+            raise RuntimeError(f"ratpoints_serve --mode {mode} failed: {proc.stderr[-2000:]}")
+        summaries[mode] = json.loads(proc.stdout.strip().splitlines()[-1])
+        with open(out) as fh:
+            points[mode] = {line.split()[0]: line.split()[4:] for line in fh}
+    same = points["serve"] == points["alone"]
+    return summaries, same
+
+
+def modes_leg(wl):
+    """Config 4: 1,024 curves at once -- 512 cubics/quartics (64-128-bit
+    coefficients) with square denominators k <= 46,340 and A = 1e7, 512 of
+    degree 50-100 with 1,024-bit coefficients at height 1e5, each with 1-3
+    x-intervals -- through the serving loop (all admitted together) and one
+    search at a time."""
+    reqs = wl.config4_requests()
+    sites = sum((2 * r["bound"] + 1) * (r["last"] - r["first"] + 1) for r in reqs)
+    summaries, same = run_serve_modes(wl, reqs, "modes")
+    serve, alone = summaries["serve"], summaries["alone"]
+    return {
+        "workload": "config 4: 1024 curves, seed 4 (512 square-denominator cubics/quartics "
+                    "k<=46340 A=1e7; 512 degree 50-100, 1024-bit, height 1e5; 1-3 intervals each)",
+        "box_sites": sites,
+        "batched_curves_per_s": round(len(reqs) / serve["makespan_s"], 1),
+        "batched_wall_s": serve["makespan_s"],
+        "batched_gsites": serve["gsites_per_s"],
+        "plain_loop_curves_per_s": round(len(reqs) / alone["makespan_s"], 1),
+        "plain_loop_wall_s": alone["makespan_s"],
+        "speedup": round(alone["makespan_s"] / max(serve["makespan_s"], 1e-9), 3),
+        "points": serve["points"],
+        "errors": serve["errors"],
+        "identical_to_plain_loop": same,
     }
 
 
@@ -221,6 +252,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched (4,096-curve) leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-modes", action="store_true",
+                    help="skip the mode-coverage (1,024-curve) leg")
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-qps", type=float, default=2.0)
     args = ap.parse_args()
@@ -325,6 +358,10 @@ def main():
         result["batched"] = batched_leg(pkg, wl)
         if not result["batched"]["identical_to_plain_loop"]:
             result["error"] = "batched survivors differ from the plain loop"
+    if not args.no_modes:
+        result["modes"] = modes_leg(wl)
+        if not result["modes"]["identical_to_plain_loop"]:
+            result["error"] = "mode-coverage batch differs from the plain loop"
     if not args.no_serve:
         result["serving"] = serving_leg(wl, args.serve_requests, args.serve_qps)
         if not result["serving"]["identical_to_alone"]:

@@ -283,3 +283,81 @@ def write_trace(path, requests, square=False, no_infinity=False):
             parts.append(str(len(r["coefficients"])))
             parts += [str(c) for c in r["coefficients"]]
             fh.write(" ".join(parts) + "\n")
+
+
+def _random_intervals(rng, count):
+    out = []
+    for _ in range(count):
+        lo = Fraction(rng.uniform(-60, 60), rng.uniform(1, 9))
+        hi = lo + Fraction(rng.uniform(1, 80), rng.uniform(1, 9))
+        out.append((lo, hi))
+    return out
+
+
+def config4_requests(seed=4, count=1024, square_fraction=0.5, square_k=46340,
+                     square_bound=10_000_000, high_height=100_000):
+    """Mode-coverage batch: `count` curves, all arriving at once.
+    * square part: cubic and quartic curves with 64-128-bit coefficients,
+      searched with square denominators b = k^2, k <= square_k, numerator
+      bound square_bound; 0-2 planted points (a, k^2) with the batched
+      construction;
+    * high-degree part: degree 50-100, random 1,024-bit coefficients, height
+      high_height (|a| <= H, b <= H), squarefree by construction check.
+    Every curve carries 1-3 closed x-intervals; points at infinity enabled.
+    Returns request dicts (see config5_requests) with key "square"."""
+    rng = SplitMix64(seed)
+    n_square = int(round(count * square_fraction))
+    out = []
+    for i in range(count):
+        if i < n_square:
+            degree = rng.uniform(3, 4)
+            bits = rng.uniform(64, 128)
+            n_points = rng.uniform(0, 2)
+            ks = [rng.uniform(1, square_k) for _ in range(n_points)]
+            coeffs, pts = batched_curve(rng, degree, bits, 0, square_bound, 1)
+            # Plant (a, k^2): rebuild with factors (k^2 x - a).
+            if n_points:
+                half = max(1, bits // 2)
+                while True:
+                    q_deg = degree // 2 if degree % 2 == 0 else (degree - 1) // 2
+                    q = [rng.uniform(-(1 << half), 1 << half) for _ in range(q_deg + 1)]
+                    if q[-1] == 0:
+                        continue
+                    r = 0
+                    while r == 0:
+                        r = rng.uniform(-100, 100)
+                    pts = []
+                    for k in ks:
+                        while True:
+                            a = rng.uniform(-square_bound, square_bound)
+                            if math.gcd(a, k * k) == 1:
+                                break
+                        pts.append((a, k * k))
+                    factors = [[-a, b] for a, b in pts]
+                    while len(factors) < degree and degree % 2 == 1:
+                        factors.append([rng.uniform(-1000, 1000), rng.uniform(1, 1000)])
+                    prod = [1]
+                    for fac in factors:
+                        prod = poly_mul(prod, fac)
+                    f = poly_add(poly_mul(q, q), [r * c for c in prod])
+                    _trim(f)
+                    if len(f) - 1 == degree and squarefree_fast(f):
+                        coeffs = f
+                        break
+            else:
+                pts = []
+            out.append({"id": i, "arrival_ms": 0.0, "coefficients": coeffs,
+                        "bound": square_bound, "first": 1, "last": square_k, "square": True,
+                        "first_point": False, "intervals": _random_intervals(rng, rng.uniform(1, 3)),
+                        "points": pts, "degree": degree})
+        else:
+            degree = rng.uniform(50, 100)
+            while True:
+                coeffs = [rng.uniform(-(1 << 1023), 1 << 1023) for _ in range(degree + 1)]
+                if coeffs[-1] != 0 and squarefree_fast(coeffs):
+                    break
+            out.append({"id": i, "arrival_ms": 0.0, "coefficients": coeffs,
+                        "bound": high_height, "first": 1, "last": high_height, "square": False,
+                        "first_point": False, "intervals": _random_intervals(rng, rng.uniform(1, 3)),
+                        "points": [], "degree": degree})
+    return out

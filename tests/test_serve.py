@@ -153,3 +153,37 @@ def test_serve_matches_alone_on_gpu(pkg, workloads, tmp_path):
     print("case: serve-gpu requests=%d" % len(reqs), flush=True)
     results = run_modes(binary, trace, tmp_path, ("--round-sites", "2e8", "--max-active", "16"))
     check(reqs, results, oracle_limit=150_000)
+
+
+def mode_trace(workloads, count, **sizes):
+    reqs = workloads.config4_requests(seed=44, count=count, **sizes)
+    for r in reqs:
+        r["arrival_ms"] = 0.0
+    return reqs
+
+
+def test_mode_coverage_batch_on_double(workloads, tmp_path):
+    # Config-4 shapes at oracle-sized boxes: square-denominator cubics and
+    # quartics (b = k^2) and degree 50-100 curves with 1,024-bit
+    # coefficients, each with 1-3 x-intervals, all submitted at once.
+    if not os.path.exists(DOUBLE):
+        pytest.skip("oracle serve double not built")
+    reqs = mode_trace(workloads, 8, square_k=40, square_bound=60, high_height=12)
+    trace = tmp_path / "trace.txt"
+    workloads.write_trace(str(trace), reqs)
+    results = run_modes(DOUBLE, trace, tmp_path, ("--round-sites", "3000", "--max-active", "3"))
+    check(reqs, results, oracle_limit=10 ** 9)
+
+
+@pytest.mark.gpu
+def test_mode_coverage_batch_on_gpu(pkg, workloads, tmp_path):
+    binary = os.path.join(os.path.dirname(pkg.LIB_PATH), "..", "bin", "ratpoints_serve")
+    reqs = mode_trace(workloads, 24, square_k=2000, square_bound=100_000, high_height=3000)
+    small = mode_trace(workloads, 8, square_k=40, square_bound=60, high_height=12)
+    for k, r in enumerate(small):
+        r["id"] = 100 + k
+    trace = tmp_path / "trace.txt"
+    workloads.write_trace(str(trace), reqs + small)
+    print("case: modes-gpu requests=%d" % (len(reqs) + len(small)), flush=True)
+    results = run_modes(binary, trace, tmp_path, ("--round-sites", "1e9", "--max-active", "32"))
+    check(reqs + small, results, oracle_limit=10_000)
