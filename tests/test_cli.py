@@ -171,6 +171,41 @@ def test_not_squarefree_message(binary):
     assert b"not squarefree" in proc.stderr
 
 
+def _degree_100_curve():
+    import random
+    rng = random.Random(2024)
+    c = [rng.randrange(-(1 << 1023), 1 << 1023) for _ in range(101)]
+    c[-1] = c[-1] or 1
+    return " ".join(map(str, c))
+
+
+SQUAREFREE_CASES = [
+    # x (x - p): a repeated root modulo the first certificate prime only.
+    ("0 -2147483647 1", True),
+    # Leading coefficient divisible by the first certificate prime.
+    ("1 0 0 6442450941", True),
+    # (x^2 + 1)^2 (x - 3) and x^3 (x + 1): repeated factors, rejected by the
+    # exact fallback.
+    ("-3 1 -6 2 -3 1", False),
+    ("0 0 0 1 1", False),
+    # Degree 100 with 1,024-bit coefficients: the certificate answers at once
+    # where Euclid over Q alone runs for minutes.
+    (_degree_100_curve(), True),
+]
+
+
+@pytest.mark.parametrize("binary", BINARIES)
+@pytest.mark.parametrize("case", SQUAREFREE_CASES, ids=[c[0][:24] for c in SQUAREFREE_CASES])
+def test_squarefree_certificate_cases(binary, case):
+    import time
+    coeffs, squarefree = case
+    t0 = time.perf_counter()
+    proc = run(binary, [coeffs, "3", "-q"], ok=squarefree)
+    assert time.perf_counter() - t0 < 30
+    if not squarefree:
+        assert b"not squarefree" in proc.stderr
+
+
 @pytest.mark.parametrize("binary", BINARIES)
 def test_dense_streaming_counts(binary):
     # tests/test_multi_gpu.py:122-130: every site of 1 + P x survives.
