@@ -31,6 +31,9 @@ BOUND = 5_000_000
 LAST = 5_000_000
 # MI355X LDS read bandwidth: 256 CUs x 128 B/clk x 2.4 GHz.
 LDS_PEAK_GBS = 256 * 128 * 2.4
+# Algorithmic LDS bytes per site, the reference's fast-stage work: 14 mask
+# words of 4 bytes per 32 numerators (src/sieve.cu:296-331) = 1.75 B/site.
+ALGORITHMIC_LDS_BYTES_PER_SITE = 14 * 4 / 32
 RTX2000_ADA_GSITES = 879.5   # published, A = B = 5e6 (README.md:40-47)
 RTX4090_GSITES = 7423.0      # published, A = B = 1e7 end to end
 
@@ -306,10 +309,10 @@ def main():
     gsites = sites / wall / 1e9
     kernel_gsites = kernel_sites / (kernel_ms / 1e3) / 1e9 if kernel_ms else 0.0
     per_launch_ms = kernel_ms / launches if launches else 0.0
-    # LDS bytes the dense stage reads per site: one 64-bit row word per dense
-    # prime per 64 numerators.
-    lds_bytes_per_site = dense_primes(variant) * 8 / 64
-    lds_bytes_per_launch = lds_bytes_per_site * kernel_sites / max(launches, 1)
+    # Roofline on the algorithmic work; the kernel itself reads one 64-bit
+    # row word per dense prime per 64 numerators (1.5 B/site at 12 primes).
+    lds_bytes_read_per_site = dense_primes(variant) * 8 / 64
+    lds_bytes_per_launch = ALGORITHMIC_LDS_BYTES_PER_SITE * kernel_sites / max(launches, 1)
     achieved = lds_bytes_per_launch / (per_launch_ms / 1e3) / 1e9 if per_launch_ms else 0.0
 
     result = {
@@ -342,7 +345,8 @@ def main():
             "traffic": measured_traffic(variant),
             "kernel": variant,
             "kernel_ms_per_launch": round(per_launch_ms, 3),
-            "bytes_per_site": lds_bytes_per_site,
+            "bytes_per_site": ALGORITHMIC_LDS_BYTES_PER_SITE,
+            "dense_lds_bytes_read_per_site": lds_bytes_read_per_site,
         },
         "kernel_gsites": round(kernel_gsites, 2),
         "survivors": int(stats["survivor_count"]),
