@@ -487,9 +487,8 @@ struct RowTileQ {
     static constexpr int kStride = Base::kStride;
     static constexpr int kSlot = Base::kSlot;
     static constexpr int kQueue = 128;
-    // Per wave: 128 masks (128 words), 128 uint32 word offsets (64 words),
-    // then the late primes' line offsets (c + 64 (ts mod p)) mod p (16 words).
-    static constexpr int kQueueWords = kQueue + kQueue / 2 + kSelected / 2;
+    // Per wave: 128 masks (128 words) and 128 uint32 word offsets (64 words).
+    static constexpr int kQueueWords = kQueue + kQueue / 2;
     static constexpr int kQueueBase = Base::kLdsWords;
     static constexpr int kLdsWords = kQueueBase + WAVES * kQueueWords;
     static_assert((kStride & (kStride - 1)) == 0, "the cursor encodes u below the stride");
@@ -510,13 +509,13 @@ struct RowTileQ {
     // ts + rel has offset (cb + 64 rel) mod p in the residue string (rel <
     // 2^25, so the sum stays below 2^32), and its window is two funnel
     // shifts of three 32-bit string words.
+    // Lane j of (lp, lmagic, lcb) holds late prime FAST + j: p, its magic and
+    // its line offset (c + 64 (ts mod p)) mod p.
     __device__ static bool drain(const uint64_t *pool, const uint64_t *qm,
-                                 const uint32_t *qw, const uint32_t *qcb,
-                                 uint32_t n, int b, uint64_t ts,
+                                 const uint32_t *qw, uint32_t lp, uint32_t lmagic,
+                                 uint32_t lcb, uint32_t n, int b, uint64_t ts,
                                  const LineDesc &L, const OutDesc &O,
                                  int request, int lane) {
-        const uint32_t *s_p = reinterpret_cast<const uint32_t *>(pool + Base::kSmall);
-        const uint32_t *s_magic = s_p + 5 * kSelected;
         const uint32_t *cext = reinterpret_cast<const uint32_t *>(pool + Base::kCext);
         uint64_t m[2];
         uint32_t rel[2];
@@ -533,9 +532,9 @@ struct RowTileQ {
             if (!__ballot((m[0] | m[1]) != 0)) {
                 break;
             }
-            const uint32_t p = uniform(s_p[s]);
-            const uint32_t magic = uniform(s_magic[s]);
-            const uint32_t cb = uniform(qcb[s - FAST]);
+            const uint32_t p = __builtin_amdgcn_readlane(lp, s - FAST);
+            const uint32_t magic = __builtin_amdgcn_readlane(lmagic, s - FAST);
+            const uint32_t cb = __builtin_amdgcn_readlane(lcb, s - FAST);
             const uint32_t *cx = cext + s * 20;
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
@@ -572,14 +571,13 @@ struct RowTileQ {
         const uint32_t *s_magic = s_base + kSelected;
         uint64_t *qm = pool + kQueueBase + wave * kQueueWords;
         uint32_t *qw = reinterpret_cast<uint32_t *>(qm + kQueue);
-        uint32_t *qcb = qw + kQueue;
+        uint32_t lp = 1, lmagic = 0, lcb = 0;
         if (lane < kSelected - FAST) {
             const uint32_t *s_c = s_p + kSelected;
-            const uint32_t p = s_p[FAST + lane];
-            qcb[lane] = mod_small(s_c[FAST + lane] + 64u * s_base[FAST + lane], p,
-                                  s_magic[FAST + lane]);
+            lp = s_p[FAST + lane];
+            lmagic = s_magic[FAST + lane];
+            lcb = mod_small(s_c[FAST + lane] + 64u * s_base[FAST + lane], lp, lmagic);
         }
-        wave_sync();
         // Iterations start at ts + k kStride, so the sub-word is the cursor's
         // offset from the tile start modulo the stride.
         const uint32_t skip0 = static_cast<uint32_t>((from - ts) & (kStride - 1));
@@ -656,14 +654,14 @@ struct RowTileQ {
                 }
                 queued += static_cast<uint32_t>(__popcll(bal));
                 if (queued >= 64) {
-                    if (!drain(pool, qm, qw, qcb, queued, b, ts, L, O, request, lane)) {
+                    if (!drain(pool, qm, qw, lp, lmagic, lcb, queued, b, ts, L, O, request, lane)) {
                         return oldest;
                     }
                     queued = 0;
                 }
             }
         }
-        if (queued && !drain(pool, qm, qw, qcb, queued, b, ts, L, O, request, lane)) {
+        if (queued && !drain(pool, qm, qw, lp, lmagic, lcb, queued, b, ts, L, O, request, lane)) {
             return oldest;
         }
         return we;
