@@ -255,8 +255,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched (4,096-curve) leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
-    ap.add_argument("--no-modes", action="store_true",
-                    help="skip the mode-coverage (1,024-curve) leg")
+    ap.add_argument("--modes", action="store_true",
+                    help="run the mode-coverage (1,024-curve) leg")
     ap.add_argument("--serve-requests", type=int, default=40)
     ap.add_argument("--serve-qps", type=float, default=2.0)
     args = ap.parse_args()
@@ -362,7 +362,7 @@ def main():
         result["batched"] = batched_leg(pkg, wl)
         if not result["batched"]["identical_to_plain_loop"]:
             result["error"] = "batched survivors differ from the plain loop"
-    if not args.no_modes:
+    if args.modes:
         result["modes"] = modes_leg(wl)
         if not result["modes"]["identical_to_plain_loop"]:
             result["error"] = "mode-coverage batch differs from the plain loop"
