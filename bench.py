@@ -192,10 +192,11 @@ def run_serve_modes(wl, reqs, label, extra=()):
     for mode in ("serve", "alone"):
         print(f"bench: {label} {mode} ({len(reqs)} requests)", file=sys.stderr, flush=True)
         out = os.path.join(work, mode + ".txt")
+        # stderr (the tool's progress lines) passes through to ours.
         proc = subprocess.run([binary, "--trace", trace, "--mode", mode, "--out", out, *extra],
-                              capture_output=True, text=True, timeout=1200)
+                              stdout=subprocess.PIPE, text=True, timeout=1200)
         if proc.returncode != 0:
-            raise RuntimeError(f"ratpoints_serve --mode {mode} failed: {proc.stderr[-2000:]}")
+            raise RuntimeError(f"ratpoints_serve --mode {mode} failed (exit {proc.returncode})")
         summaries[mode] = json.loads(proc.stdout.strip().splitlines()[-1])
         with open(out) as fh:
             points[mode] = {line.split()[0]: line.split()[4:] for line in fh}

@@ -11,6 +11,7 @@
 // Output (--out): per request "id latency_ms sites rounds n_points
 // a:y:b ..." or "id error <message>"; stdout: one JSON summary line.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -160,6 +161,20 @@ int main(int argc, char **argv) {
             std::this_thread::sleep_for(std::chrono::duration<double, std::milli>(ms - now));
         }
     };
+    // Progress on stderr every 10 s: requests finished so far.
+    std::atomic<size_t> finished{0};
+    std::atomic<bool> running{true};
+    std::thread progress([&] {
+        double next = 10000.0;
+        while (running.load()) {
+            std::this_thread::sleep_for(std::chrono::milliseconds(200));
+            if (since_start() >= next) {
+                std::fprintf(stderr, "ratpoints_serve: %s %.0f s, %zu of %zu requests done\n",
+                             mode.c_str(), since_start() / 1e3, finished.load(), trace.size());
+                next += 10000.0;
+            }
+        }
+    });
     ServeStats stats;
     if (mode == "serve") {
         Server server(options, [&](const ServeResult &r) {
@@ -167,6 +182,7 @@ int main(int argc, char **argv) {
             ServeResult done = r;
             done.finish_ms = since_start();
             results[r.id] = std::move(done);
+            ++finished;
         });
         for (const TraceEntry &e : trace) {
             const double at = e.arrival_ms * time_scale;
@@ -189,9 +205,12 @@ int main(int argc, char **argv) {
             stats.sites += r.sites;
             stats.rounds += static_cast<unsigned long long>(r.rounds);
             results[r.id] = std::move(r);
+            ++finished;
         }
     }
     const double makespan_ms = since_start();
+    running = false;
+    progress.join();
 
     std::vector<double> latency;
     unsigned long long errors = 0, points = 0;
