@@ -258,8 +258,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--modes", action="store_true",
                     help="run the mode-coverage (1,024-curve) leg")
-    ap.add_argument("--serve-requests", type=int, default=40)
-    ap.add_argument("--serve-qps", type=float, default=2.0)
+    ap.add_argument("--serve-requests", type=int, default=100)
+    ap.add_argument("--serve-qps", type=float, default=5.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; one MI355X covers the whole "
