@@ -251,7 +251,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=77)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="kernel variant index; -1 (default) = the tuning table's choice")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched (4,096-curve) leg")
@@ -276,7 +277,8 @@ def main():
     ctx = pkg.SieveContext(0)
     ctx.set_option("variant", args.variant)
     ctx.set_option("batch_size", BATCH)
-    variant = pkg.variants()[args.variant]
+    words = (2 * BOUND + 64) // 64
+    variant = pkg.variants()[args.variant if args.variant >= 0 else pkg.tuned_variant(words)]
 
     # Warmup: the first `warmup` batches (also builds the residue table).
     warm_last = min(LAST, max(1, args.warmup) * BATCH)

@@ -160,7 +160,8 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests,
  * compile-time macros (src/sieve_plan.hpp:10-22, src/sieve.cu:15-17):
  *   "survivor_capacity"  survivors per flush buffer, >= 16384 (default 2^24)
  *   "batch_size"         denominators per launch, 1..1048576 (default 65536)
- *   "variant"            kernel variant index, 0..rps_variant_count()-1
+ *   "variant"            kernel variant index, 0..rps_variant_count()-1, or
+ *                        -1 (default): chosen per request by rps_tuned_variant
  *   "min_tiles"          split denominators into word chunks until at least
  *                        this many tiles per launch exist (default 2048)
  *   "direct_words"       lines of at most this many 64-bit words use the
@@ -177,6 +178,11 @@ int rps_set_option(rps_ctx *ctx, const char *name, int64_t value);
  * dense primes, waves per workgroup).  Variant 0 is the default. */
 int rps_variant_count(void);
 const char *rps_variant_name(int variant);
+
+/* The variant the autotuner's table picks for lines of `words` 64-bit words
+ * (used when the "variant" option is -1, the default).  No counterpart in the
+ * reference, whose kernel has one configuration. */
+int rps_tuned_variant(uint64_t words);
 
 /* Device enumeration for --list-devices (include/gpu_devices.hpp:20-21,
  * src/gpu_devices.cpp:55-97).  rps_device_info writes a NUL-terminated name

@@ -105,6 +105,8 @@ def load_library(path=LIB_PATH):
     lib.rps_variant_count.argtypes = []
     lib.rps_variant_name.restype = ctypes.c_char_p
     lib.rps_variant_name.argtypes = [ctypes.c_int]
+    lib.rps_tuned_variant.restype = ctypes.c_int
+    lib.rps_tuned_variant.argtypes = [ctypes.c_uint64]
     lib.rps_device_count.restype = ctypes.c_int
     lib.rps_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
     lib.rps_last_kernel_timing.restype = ctypes.c_int
@@ -118,6 +120,12 @@ def load_library(path=LIB_PATH):
 def variants():
     lib = load_library()
     return [lib.rps_variant_name(i).decode() for i in range(lib.rps_variant_count())]
+
+
+def tuned_variant(words):
+    """Index of the variant the tuning table picks for lines of `words`
+    64-bit words."""
+    return load_library().rps_tuned_variant(int(words))
 
 
 class SieveResult:

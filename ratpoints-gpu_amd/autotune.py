@@ -1,0 +1,89 @@
+"""Autotuner: picks the row-tile kernel variant per line-length bucket.
+
+Runs on a GPU.  For each bucket of line lengths (64-bit words per
+denominator line) it times every compiled variant on the plain path over
+one representative line length (a planted sextic, about 3e11 sites, runs of
+lines on), and writes ``csrc/tuning_table.inc`` -- the table the library
+uses to choose a variant for a request when the caller has not forced one
+with ``rps_set_option("variant")``.  Rebuild the library afterwards.
+
+    python ratpoints-gpu_amd/autotune.py            # measure and write
+    python ratpoints-gpu_amd/autotune.py --dry-run  # measure and print
+"""
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, "csrc", "tuning_table.inc")
+# Upper ends of the buckets (words per line) and the length timed for each.
+BUCKETS = [(1536, 1024), (3072, 2048), (6144, 4096), (12288, 8192),
+           (24576, 16384), (49152, 32768), (1 << 62, 156251)]
+
+
+def load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    module = importlib.util.module_from_spec(spec)
+    sys.modules[name] = module
+    spec.loader.exec_module(module)
+    return module
+
+
+def measure(sites_per_point=3e11):
+    pkg = load("ratpoints_gpu_amd", os.path.join(HERE, "__init__.py"))
+    wl = load("ratpoints_gpu_amd_workloads", os.path.join(HERE, "workloads.py"))
+    coeffs = [str(c) for c in wl.config2_curve()[0]]
+    names = pkg.variants()
+    rows = []
+    with pkg.SieveContext(0) as ctx:
+        for upper, words in BUCKETS:
+            bound = 32 * words - 32
+            lines = max(65536, int(sites_per_point / (2 * bound + 1)) // 65536 * 65536)
+            rates = {}
+            for v, name in enumerate(names):
+                ctx.set_option("variant", v)
+                ctx.run(coeffs, bound, 1, 65536, False, lambda n, d: False)
+                ctx.run(coeffs, bound, 1, lines, False, lambda n, d: False)
+                ms, _, sites = ctx.kernel_timing()
+                rates[name] = sites / ms / 1e6
+            best = max(rates, key=rates.get)
+            row = {"max_words": upper, "words": words, "best": best,
+                   "gsites": {k: round(r, 1) for k, r in rates.items()}}
+            print(json.dumps(row), flush=True)
+            rows.append(row)
+    return rows
+
+
+def write_table(rows, path=TABLE):
+    lines = [
+        "// Generated by ratpoints-gpu_amd/autotune.py on one MI355X (%s);" % time.strftime("%Y-%m-%d"),
+        "// kernel Gsite/s of every variant per line-length bucket, the fastest chosen.",
+        "// {largest words per line of the bucket, variant name}",
+    ]
+    for row in rows:
+        rates = ", ".join("%s %.0f" % (k, v) for k, v in sorted(row["gsites"].items()))
+        lines.append("// <= %d words (timed at %d): %s" % (min(row["max_words"], 10 ** 12),
+                                                           row["words"], rates))
+    for row in rows:
+        upper = "~0ULL" if row["max_words"] >= 1 << 62 else "%dULL" % row["max_words"]
+        lines.append('{%s, "%s"},' % (upper, row["best"]))
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--dry-run", action="store_true")
+    args = ap.parse_args()
+    rows = measure()
+    if not args.dry_run:
+        write_table(rows)
+        print("wrote", TABLE)
+
+
+if __name__ == "__main__":
+    main()

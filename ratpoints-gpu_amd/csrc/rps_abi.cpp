@@ -182,7 +182,7 @@ struct rps_ctx {
     int *h_req = nullptr;
     size_t stage_capacity = size_t(1) << 20;
     int batch_size = 1 << 16;
-    int variant = 0;
+    int variant = -1;  // -1: chosen per request by the tuning table
     long long min_tiles = 2048;
     unsigned long long direct_words = rps::kDirectMaxWords;
     hipEvent_t basis_start = nullptr, basis_end = nullptr;
@@ -278,6 +278,32 @@ void validate(const rps_request *req) {
         || (req->square_denominators && req->den_last > rps::kMaxParamT)) {
         throw ApiError{RPS_EINVAL, "invalid denominator range"};
     }
+}
+
+// Variant per line length from the autotuner's table (autotune.py writes
+// tuning_table.inc); the first entry whose bucket holds `words`, or variant 0
+// when a name in the table is not compiled in.
+struct TuneEntry {
+    unsigned long long max_words;
+    const char *variant;
+};
+
+const TuneEntry kTuning[] = {
+#include "tuning_table.inc"
+};
+
+int tuned_variant(unsigned long long words) {
+    for (const TuneEntry &t : kTuning) {
+        if (words <= t.max_words) {
+            for (int v = 0; v < rps::variant_count(); ++v) {
+                if (std::strcmp(rps::variant(v)->name, t.variant) == 0) {
+                    return v;
+                }
+            }
+            return 0;
+        }
+    }
+    return 0;
 }
 
 // Line geometry of a request: words per line, last-word mask, and the
@@ -531,7 +557,7 @@ int rps_set_option(rps_ctx *ctx, const char *name, int64_t value) {
         }
         ctx->batch_size = static_cast<int>(value);
     } else if (key == "variant") {
-        if (value < 0 || value >= rps::variant_count()) {
+        if (value < -1 || value >= rps::variant_count()) {
             ctx->error = "variant out of range";
             return RPS_EINVAL;
         }
@@ -562,6 +588,8 @@ int rps_set_option(rps_ctx *ctx, const char *name, int64_t value) {
 }
 
 int rps_variant_count(void) { return rps::variant_count(); }
+
+int rps_tuned_variant(uint64_t words) { return tuned_variant(words); }
 
 const char *rps_variant_name(int index) {
     const rps::VariantInfo *v = rps::variant(index);
@@ -643,9 +671,10 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
         }
         validate(req);
         RPS_HIP(hipSetDevice(ctx->device));
-        const rps::VariantInfo *var = rps::variant(ctx->variant);
-        const int stride = 64 * var->words_per_lane;
         const rps::LineDesc geometry = line_geometry(req);
+        const rps::VariantInfo *var = rps::variant(
+            ctx->variant >= 0 ? ctx->variant : tuned_variant(geometry.words));
+        const int stride = 64 * var->words_per_lane;
         const unsigned long long words = geometry.words;
         const long long A = req->numerator_bound;
         st.word_count = (2ULL * static_cast<unsigned long long>(A) + 32) / 32;
@@ -868,7 +897,24 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
             n_coef = std::max(n_coef, requests[r].n_coefficients);
         }
         RPS_HIP(hipSetDevice(ctx->device));
-        const rps::VariantInfo *var = rps::variant(ctx->variant);
+        // One variant per launch: the forced one, or the tuning table's
+        // choice for the request with the most sites.
+        int chosen = ctx->variant;
+        if (chosen < 0) {
+            int largest = 0;
+            double most = -1.0;
+            for (int r = 0; r < n; ++r) {
+                const rps::LineDesc g = line_geometry(&requests[r]);
+                const double work = (2.0 * static_cast<double>(g.bound) + 1.0)
+                    * (static_cast<double>(requests[r].den_last) - requests[r].den_first + 1.0);
+                if (work > most) {
+                    most = work;
+                    largest = r;
+                }
+            }
+            chosen = tuned_variant(line_geometry(&requests[largest]).words);
+        }
+        const rps::VariantInfo *var = rps::variant(chosen);
         const int stride = 64 * var->words_per_lane;
 
         // Geometry and tiling per curve.  Long lines are cut into chunks of

@@ -47,6 +47,18 @@ def test_variants_listed(pkg):
         assert re.fullmatch(r"(q_)?u\d_f\d+_w\d+", n)
 
 
+def test_tuning_table_names_compiled_variants(pkg):
+    # Every variant the autotuner's table names is compiled in, and the
+    # choice is defined for any line length.
+    import os
+    names = pkg.variants()
+    table = os.path.join(os.path.dirname(pkg.LIB_PATH), "..", "csrc", "tuning_table.inc")
+    listed = re.findall(r'\{[^,]+, "([^"]+)"\}', open(table).read())
+    assert listed and all(n in names for n in listed)
+    for words in (1, 769, 2048, 10 ** 5, 3 * 10 ** 10):
+        assert 0 <= pkg.tuned_variant(words) < len(names)
+
+
 def test_library_targets_gfx950(pkg):
     out = subprocess.run(["strings", pkg.LIB_PATH], capture_output=True, text=True).stdout
     assert "gfx950" in out
