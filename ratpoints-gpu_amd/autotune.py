@@ -78,11 +78,12 @@ def write_table(rows, path=TABLE):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--dry-run", action="store_true")
+    ap.add_argument("--out", default=TABLE, help="table path (default: csrc/tuning_table.inc)")
     args = ap.parse_args()
     rows = measure()
     if not args.dry_run:
-        write_table(rows)
-        print("wrote", TABLE)
+        write_table(rows, args.out)
+        print("wrote", args.out)
 
 
 if __name__ == "__main__":
