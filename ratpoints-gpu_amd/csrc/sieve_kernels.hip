@@ -598,73 +598,113 @@ struct RowTileQ {
                 prime[s] = uniform(s_p[s]);
             }
         }
+        // Iterations whose words all lie below min(we, last word) need no
+        // masking; they run the unmasked body.  The first iteration of a
+        // resumed sweep and the range's tail run the masked one.
         const uint64_t last_word = L.words - 1;
-        uint32_t queued = 0;
-        uint64_t oldest = ws;
-        uint32_t skip = skip0;
-        for (uint64_t W = ws; W < we; W += kStride) {
-            uint64_t m[U];
-            Base::dense(pool, off, prime, lane, m);
-            if (W + kStride >= we) {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint64_t w = W + lane + 64 * u;
-                    if (w >= we) {
-                        m[u] = 0;
-                    } else if (w == last_word) {
-                        m[u] &= L.last_mask;
-                    }
-                }
+        const uint32_t n_iter = static_cast<uint32_t>((we - ws + kStride - 1) / kStride);
+        const uint64_t clean_end = we < last_word ? we : last_word;
+        uint32_t n_clean = clean_end > ws
+            ? static_cast<uint32_t>((clean_end - ws) / kStride) : 0u;
+        n_clean = n_clean < n_iter ? n_clean : n_iter;
+        Queue q{qm, qw, 0u, ws};
+        const Late late{lp, lmagic, lcb};
+        uint32_t it = 0;
+        if (skip0 && n_iter) {
+            if (!step<true>(pool, ws, we, last_word, skip0, off, prime, q, late, b, ts, L, O,
+                            request, lane)) {
+                return q.oldest;
             }
-            if (skip) {
-                // First iteration of a resumed sweep: sub-words below the
-                // cursor were committed before.
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    if (static_cast<uint32_t>(u) < skip) {
-                        m[u] = 0;
-                    }
-                }
-                skip = 0;
-            }
-            uint64_t any = 0;
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                any |= m[u];
-            }
-            if (!__ballot(any != 0)) {
-                continue;
-            }
-            const uint32_t rel = static_cast<uint32_t>(W - ts) + lane;
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint64_t bal = __ballot(m[u] != 0);
-                if (!bal) {
-                    continue;
-                }
-                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-                    static_cast<uint32_t>(bal >> 32),
-                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bal), 0u));
-                if (queued == 0) {
-                    oldest = W + u;
-                }
-                if (m[u]) {
-                    qm[queued + rank] = m[u];
-                    qw[queued + rank] = rel + 64u * u;
-                }
-                queued += static_cast<uint32_t>(__popcll(bal));
-                if (queued >= 64) {
-                    if (!drain(pool, qm, qw, lp, lmagic, lcb, queued, b, ts, L, O, request, lane)) {
-                        return oldest;
-                    }
-                    queued = 0;
-                }
+            it = 1;
+        }
+        for (; it < n_clean; ++it) {
+            if (!step<false>(pool, ws + static_cast<uint64_t>(it) * kStride, we, last_word, 0u,
+                             off, prime, q, late, b, ts, L, O, request, lane)) {
+                return q.oldest;
             }
         }
-        if (queued && !drain(pool, qm, qw, lp, lmagic, lcb, queued, b, ts, L, O, request, lane)) {
-            return oldest;
+        for (; it < n_iter; ++it) {
+            if (!step<true>(pool, ws + static_cast<uint64_t>(it) * kStride, we, last_word, 0u,
+                            off, prime, q, late, b, ts, L, O, request, lane)) {
+                return q.oldest;
+            }
+        }
+        if (q.queued && !drain(pool, q.qm, q.qw, late.p, late.magic, late.cb, q.queued, b, ts,
+                               L, O, request, lane)) {
+            return q.oldest;
         }
         return we;
+    }
+
+    struct Queue {
+        uint64_t *qm;
+        uint32_t *qw;
+        uint32_t queued;
+        uint64_t oldest;  // cursor (W + u) of the oldest queued entry
+    };
+
+    struct Late {
+        uint32_t p, magic, cb;  // lane j: late prime FAST + j
+    };
+
+    // One iteration at W: dense stage, (EDGE: masks for the range end, the
+    // line's last word and the first `skip` sub-words), queue pushes and,
+    // when 64 or more entries wait, a drain.  False when a drain's survivors
+    // did not fit the buffer.
+    template <bool EDGE>
+    __device__ static bool step(uint64_t *pool, uint64_t W, uint64_t we, uint64_t last_word,
+                                uint32_t skip, uint32_t (&off)[FAST],
+                                const uint32_t (&prime)[FAST], Queue &q, const Late &late,
+                                int b, uint64_t ts, const LineDesc &L, const OutDesc &O,
+                                int request, int lane) {
+        uint64_t m[U];
+        Base::dense(pool, off, prime, lane, m);
+        if (EDGE) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t w = W + lane + 64 * u;
+                if (w >= we || static_cast<uint32_t>(u) < skip) {
+                    m[u] = 0;
+                } else if (w == last_word) {
+                    m[u] &= L.last_mask;
+                }
+            }
+        }
+        uint64_t any = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            any |= m[u];
+        }
+        if (!__ballot(any != 0)) {
+            return true;
+        }
+        const uint32_t rel = static_cast<uint32_t>(W - ts) + lane;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t bal = __ballot(m[u] != 0);
+            if (!bal) {
+                continue;
+            }
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                static_cast<uint32_t>(bal >> 32),
+                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bal), 0u));
+            if (q.queued == 0) {
+                q.oldest = W + u;
+            }
+            if (m[u]) {
+                q.qm[q.queued + rank] = m[u];
+                q.qw[q.queued + rank] = rel + 64u * u;
+            }
+            q.queued += static_cast<uint32_t>(__popcll(bal));
+            if (q.queued >= 64) {
+                if (!drain(pool, q.qm, q.qw, late.p, late.magic, late.cb, q.queued, b, ts, L,
+                           O, request, lane)) {
+                    return false;
+                }
+                q.queued = 0;
+            }
+        }
+        return true;
     }
 };
 
