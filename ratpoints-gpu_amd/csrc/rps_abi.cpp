@@ -1124,7 +1124,11 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
         M.out.capacity = ctx->capacity;
         M.primes = ctx->d_primes;
         M.n_curves = n;
-        const unsigned grid = static_cast<unsigned>(ctx->cu_count) * 2u;
+        // One workgroup per resident slot: the variant's occupancy per CU.
+        int per_cu = 0;
+        RPS_HIP(var->multi_blocks_per_cu(&per_cu));
+        const unsigned grid = static_cast<unsigned>(ctx->cu_count)
+            * static_cast<unsigned>(std::max(1, per_cu));
         std::vector<std::vector<long long>> kept_num(n);
         std::vector<std::vector<int>> kept_den(n);
         unsigned long long head = 0;

@@ -1410,15 +1410,22 @@ hipError_t launch_multi_variant(const MultiParams &params, unsigned grid,
     return hipGetLastError();
 }
 
+template <class Tile, int WAVES>
+hipError_t multi_occupancy(int *blocks) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, multi_kernel<Tile, WAVES>,
+                                                        WAVES * 64, 0);
+}
+
 // "u2_f14_w8": RowTile with U = 2, FAST = 14, 8 waves; "q_u2_f12_w8": the
 // same shape with the compaction-queue late stage.
 #define RPS_VARIANT(U, F, W)                                               \
     {"u" #U "_f" #F "_w" #W, U, F, W, launch_variant<RowTile<U, F, W>, W>, \
-     launch_multi_variant<RowTile<U, F, W>, W>}
+     launch_multi_variant<RowTile<U, F, W>, W>, multi_occupancy<RowTile<U, F, W>, W>}
 #define RPS_VARIANT_Q(U, F, W)                                             \
     {"q_u" #U "_f" #F "_w" #W, U, F, W,                                   \
      launch_variant<RowTileQ<U, F, W>, W>,                                 \
-     launch_multi_variant<RowTileQ<U, F, W>, W>}
+     launch_multi_variant<RowTileQ<U, F, W>, W>,                           \
+     multi_occupancy<RowTileQ<U, F, W>, W>}
 
 const VariantInfo kVariants[] = {
     RPS_VARIANT_Q(2, 12, 8),
@@ -1427,6 +1434,11 @@ const VariantInfo kVariants[] = {
     RPS_VARIANT_Q(1, 12, 8),
     RPS_VARIANT_Q(2, 12, 16),
     RPS_VARIANT(2, 14, 8),
+    // Small-LDS tiles (3-4 workgroups per CU) for short lines, where the
+    // per-line set-up dominates.
+    RPS_VARIANT(2, 8, 8),
+    RPS_VARIANT(1, 10, 8),
+    RPS_VARIANT(1, 8, 8),
 };
 
 #undef RPS_VARIANT

@@ -18,6 +18,8 @@ struct VariantInfo {
                          hipStream_t stream);
     hipError_t (*launch_multi)(const MultiParams &params, unsigned grid,
                                hipStream_t stream);
+    // Workgroups of the persistent kernel resident per CU.
+    hipError_t (*multi_blocks_per_cu)(int *blocks);
 };
 
 int variant_count();
