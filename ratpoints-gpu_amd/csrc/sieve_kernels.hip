@@ -1434,11 +1434,6 @@ const VariantInfo kVariants[] = {
     RPS_VARIANT_Q(1, 12, 8),
     RPS_VARIANT_Q(2, 12, 16),
     RPS_VARIANT(2, 14, 8),
-    // Small-LDS tiles (3-4 workgroups per CU) for short lines, where the
-    // per-line set-up dominates.
-    RPS_VARIANT(2, 8, 8),
-    RPS_VARIANT(1, 10, 8),
-    RPS_VARIANT(1, 8, 8),
 };
 
 #undef RPS_VARIANT

@@ -207,39 +207,19 @@ struct Server::Impl {
         }
     }
 
-    // Sites each active request may sieve this round: round_sites shared by
-    // water-filling, so what small requests do not need goes to the others.
-    std::vector<double> shares() const {
-        const size_t n = active.size();
-        std::vector<double> demand(n), share(n, 0.0);
-        std::vector<size_t> order(n);
-        for (size_t i = 0; i < n; ++i) {
-            const Active &a = *active[i];
-            demand[i] = line_sites(a.request) * static_cast<double>(a.last_t - a.next_t + 1);
-            order[i] = i;
-        }
-        std::sort(order.begin(), order.end(),
-                  [&](size_t x, size_t y) { return demand[x] < demand[y]; });
-        double left = options.round_sites;
-        for (size_t k = 0; k < n; ++k) {
-            const size_t i = order[k];
-            share[i] = std::min(demand[i], left / static_cast<double>(n - k));
-            left -= share[i];
-        }
-        return share;
-    }
-
     // One launch over a chunk of every active request's denominators.
     void round() {
         const size_t n = active.size();
-        const std::vector<double> share = shares();
+        // Equal shares: a newcomer's first chunk waits at most one round,
+        // and small requests finish in the round they join.
+        const double share = options.round_sites / static_cast<double>(n);
         std::vector<rps_request> reqs(n);
         double sites = 0.0;
         for (size_t i = 0; i < n; ++i) {
             Active &a = *active[i];
             const double per_line = line_sites(a.request);
             const long long left = a.last_t - a.next_t + 1;
-            const long long want = static_cast<long long>(std::floor(share[i] / per_line));
+            const long long want = static_cast<long long>(std::floor(share / per_line));
             const long long k = std::max(1LL, std::min(left, want));
             a.chunk_last = a.next_t + k - 1;
             reqs[i] = to_request(a, a.next_t, a.chunk_last);

@@ -361,3 +361,24 @@ def config4_requests(seed=4, count=1024, square_fraction=0.5, square_k=46340,
                         "first_point": False, "intervals": _random_intervals(rng, rng.uniform(1, 3)),
                         "points": [], "degree": degree})
     return out
+
+
+def main(argv=None):
+    """Writes a serving trace: python workloads.py config5 OUT [COUNT QPS]
+    or python workloads.py config4 OUT."""
+    import sys
+    args = list(sys.argv[1:] if argv is None else argv)
+    if len(args) < 2 or args[0] not in ("config4", "config5"):
+        raise SystemExit("usage: workloads.py config5 OUT [COUNT QPS] | config4 OUT")
+    if args[0] == "config5":
+        count = int(args[2]) if len(args) > 2 else 100
+        qps = float(args[3]) if len(args) > 3 else 5.0
+        reqs = config5_requests(count=count, qps=qps)
+    else:
+        reqs = config4_requests()
+    write_trace(args[1], reqs)
+    print("wrote %d requests to %s" % (len(reqs), args[1]))
+
+
+if __name__ == "__main__":
+    main()
