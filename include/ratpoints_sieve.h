@@ -166,6 +166,10 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests,
  *                        this many tiles per launch exist (default 2048)
  *   "direct_words"       lines of at most this many 64-bit words use the
  *                        short-line kernel, 0..768 (default 768; 0 = never)
+ *   "primitive_only"     1: the device drops survivors with gcd(a, b) > 1
+ *                        before they are copied (they can never be points;
+ *                        the exact stage, src/point_search.cpp:48-60, skips
+ *                        them).  Default 0: the raw modular survivor set.
  *   "table_slots"        residue tables (1.33 MB each) kept on the device
  *                        between rps_sieve_many calls, least recently used
  *                        replaced, 1..65536 (default 256); a call with more

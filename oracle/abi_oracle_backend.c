@@ -30,6 +30,7 @@ int rps_check_request(const rps_request *req, char *message, size_t message_len)
 struct rps_ctx {
     char error[256];
     int batch_size;
+    int primitive;
 };
 
 static char open_error[256];
@@ -70,8 +71,12 @@ int rps_set_option(rps_ctx *ctx, const char *name, int64_t value) {
         ctx->batch_size = (int)value;
         return RPS_OK;
     }
+    if (strcmp(name, "primitive_only") == 0 && (value == 0 || value == 1)) {
+        ctx->primitive = (int)value;
+        return RPS_OK;
+    }
     if (strcmp(name, "survivor_capacity") == 0 || strcmp(name, "variant") == 0
-        || strcmp(name, "min_tiles") == 0) {
+        || strcmp(name, "min_tiles") == 0 || strcmp(name, "table_slots") == 0) {
         return RPS_OK;
     }
     snprintf(ctx->error, sizeof ctx->error, "unknown option: %s", name);
@@ -161,6 +166,26 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
             snprintf(ctx->error, sizeof ctx->error, "oracle sieve failed");
             status = RPS_EINTERNAL;
             break;
+        }
+        if (ctx->primitive) {
+            /* Keep gcd(a, b) = 1 only. */
+            long long kept = 0;
+            for (long long i = 0; i < n; ++i) {
+                unsigned long long x = num[i] < 0 ? 0ULL - (unsigned long long)num[i]
+                                                   : (unsigned long long)num[i];
+                unsigned long long y = (unsigned long long)den[i];
+                while (y) {
+                    unsigned long long r = x % y;
+                    x = y;
+                    y = r;
+                }
+                if (x == 1) {
+                    num[kept] = num[i];
+                    den[kept] = den[i];
+                    ++kept;
+                }
+            }
+            n = kept;
         }
         /* Reverse order, flushes of at most 97. */
         for (long long i = 0; i < n / 2; ++i) {

@@ -185,6 +185,7 @@ struct rps_ctx {
     int variant = -1;  // -1: chosen per request by the tuning table
     long long min_tiles = 2048;
     unsigned long long direct_words = rps::kDirectMaxWords;
+    int primitive = 0;  // emit only survivors with gcd(a, b) = 1
     hipEvent_t basis_start = nullptr, basis_end = nullptr;
     double last_kernel_ms = 0.0;
     uint64_t last_launches = 0;
@@ -568,6 +569,12 @@ int rps_set_option(rps_ctx *ctx, const char *name, int64_t value) {
             return RPS_EINVAL;
         }
         ctx->direct_words = static_cast<unsigned long long>(value);
+    } else if (key == "primitive_only") {
+        if (value != 0 && value != 1) {
+            ctx->error = "primitive_only must be 0 or 1";
+            return RPS_EINVAL;
+        }
+        ctx->primitive = static_cast<int>(value);
     } else if (key == "table_slots") {
         if (value < 1 || value > (1 << 16)) {
             ctx->error = "table_slots must be in 1..65536";
@@ -699,6 +706,7 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
         base.line.neg_bound_mod = ctx->d_negmod;
         base.primes = ctx->d_primes;
         base.out.capacity = ctx->capacity;
+        base.out.primitive = ctx->primitive;
 
         const long long last_t = req->den_last;
         long long next_t = req->den_first;
@@ -1122,6 +1130,7 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
         M.out.req = ctx->m_req.ptr;
         M.out.state = s.d_state;
         M.out.capacity = ctx->capacity;
+        M.out.primitive = ctx->primitive;
         M.primes = ctx->d_primes;
         M.n_curves = n;
         // One workgroup per resident slot: the variant's occupancy per CU.

@@ -80,17 +80,67 @@ __device__ __forceinline__ int denominator(const LineDesc &L, long long t) {
     return static_cast<int>(L.square ? t * t : t);
 }
 
+// gcd(|a|, b) == 1 for b >= 1 (Euclid on a mod b, then 32-bit binary gcd).
+__device__ __forceinline__ bool coprime(long long a, uint32_t b) {
+    const uint64_t ua = a < 0 ? 0ULL - static_cast<uint64_t>(a) : static_cast<uint64_t>(a);
+    uint32_t x = static_cast<uint32_t>(ua % b);
+    uint32_t y = b;
+    if (x == 0) {
+        return y == 1;
+    }
+    const int shift = __builtin_ctz(x | y);
+    x >>= __builtin_ctz(x);
+    do {
+        y >>= __builtin_ctz(y);
+        if (x > y) {
+            const uint32_t t = x;
+            x = y;
+            y = t;
+        }
+        y -= x;
+    } while (y);
+    return (x << shift) == 1;
+}
+
+// The primitive survivors of a word: bit j of m kept when gcd(a0 + j, b) = 1
+// (src/point_search.cpp:48-60 applies the same test on the host; points
+// need a primitive (a, b)).
+__device__ __forceinline__ uint64_t primitive_bits(uint64_t m, uint64_t a0, int b) {
+    uint64_t keep = 0;
+    while (m) {
+        const int j = __ffsll(static_cast<long long>(m)) - 1;
+        m &= m - 1;
+        if (coprime(static_cast<long long>(a0 + j), static_cast<uint32_t>(b))) {
+            keep |= 1ULL << j;
+        }
+    }
+    return keep;
+}
+
 // Wave-level survivor emission.  Lane l holds N masks; mask k covers the 64
 // numerators of word word[k] on denominator b.  A prefix sum of per-lane
 // counts gives every lane its slots under one atomic reservation per wave,
 // all or nothing: when the reservation does not fit, first_fail records its
 // start, nothing is written and false is returned, so the wave can stop and
 // resume at the same point after the host drains the buffer.
+// With O.primitive set, only survivors with gcd(a, b) = 1 are emitted.
 template <int N>
 __device__ __forceinline__ bool emit_words(const OutDesc &O, long long bound,
-                                           const uint64_t (&m)[N],
+                                           const uint64_t (&m_in)[N],
                                            const uint64_t (&word)[N], int b,
                                            int request, int lane) {
+    uint64_t m[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        m[k] = m_in[k];
+    }
+    if (O.primitive) {
+        const uint64_t neg_a = static_cast<uint64_t>(-bound);
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            m[k] = m[k] ? primitive_bits(m[k], neg_a + 64 * word[k], b) : 0ULL;
+        }
+    }
     uint32_t cnt = 0;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
@@ -105,6 +155,9 @@ __device__ __forceinline__ bool emit_words(const OutDesc &O, long long bound,
         }
     }
     const uint32_t total = __shfl(incl, 63, 64);
+    if (total == 0) {
+        return true;  // every survivor filtered out: nothing to reserve
+    }
     unsigned long long start = 0;
     if (lane == 0) {
         start = atomicAdd(&O.state->count, static_cast<unsigned long long>(total));

@@ -74,6 +74,8 @@ struct OutDesc {
     int *req;                            // request index per survivor, or null
     DeviceState *state;
     unsigned long long capacity;
+    int primitive;                       // drop survivors with gcd(a, b) > 1
+    int pad;
 };
 
 // Plain launch: one curve, one denominator batch, one workgroup per tile.

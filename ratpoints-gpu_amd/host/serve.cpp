@@ -277,6 +277,8 @@ struct Server::Impl {
             ctx = nullptr;
         } else {
             rps_set_option(ctx, "table_slots", std::max(1, options.max_active));
+            // Non-primitive (a, b) are never points: drop them on the device.
+            rps_set_option(ctx, "primitive_only", 1);
         }
         for (;;) {
             {

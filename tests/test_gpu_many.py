@@ -6,6 +6,7 @@ request whose survivors overflow the shared buffer is searched again alone
 and still comes out exact.
 """
 
+import math
 import random
 
 import numpy as np
@@ -148,3 +149,39 @@ def test_many_table_cache_reuse_and_eviction(pkg):
             for r, (num, den) in zip(part, results):
                 pn, pd = plain(ctx, r)
                 assert np.array_equal(num, pn) and np.array_equal(den, pd)
+
+
+def primitive(num, den):
+    keep = np.array([math.gcd(int(a), int(b)) == 1 for a, b in zip(num, den)], dtype=bool)
+    return num[keep], den[keep]
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_primitive_only_filters_on_device(fresh_gpu, oracle_lib, pkg, seed):
+    # With primitive_only set, the device drops (a, b) with gcd(a, b) > 1:
+    # the survivors equal the full survivor set filtered by gcd on the host,
+    # on the plain path for every variant and on the many-curve path.
+    reqs = random_requests(40 + seed, 16, max_bound=20000, max_den=200)
+    full = [plain(fresh_gpu, r) for r in reqs]
+    want = [primitive(n, d) for n, d in full]
+    assert sum(len(n) for n, _ in full) > sum(len(n) for n, _ in want)
+    variants = pkg.variants()
+    fresh_gpu.set_option("primitive_only", 1)
+    for v in range(len(variants)):
+        fresh_gpu.set_option("variant", v)
+        announce(case="primitive-plain", seed=seed, variant=variants[v])
+        for r, (wn, wd) in zip(reqs, want):
+            num, den = plain(fresh_gpu, r)
+            assert np.array_equal(num, wn) and np.array_equal(den, wd)
+    fresh_gpu.set_option("variant", -1)
+    announce(case="primitive-many", seed=seed, n=len(reqs))
+    results, stats = fresh_gpu.sieve_many(reqs)
+    for r, (num, den), (wn, wd) in zip(reqs, results, want):
+        assert np.array_equal(num, wn) and np.array_equal(den, wd)
+    for r, (wn, wd) in zip(reqs[:4], want[:4]):
+        if (2 * r["bound"] + 1) * (r["last"] - r["first"] + 1) < 3e7:
+            count, digest, on, od = oracle_lib.sieve(r["coefficients"], r["bound"], r["first"],
+                                                     r["last"], r.get("square", False),
+                                                     method="words")
+            pn, pd = primitive(on, od)
+            assert np.array_equal(wn, pn) and np.array_equal(wd, pd)
