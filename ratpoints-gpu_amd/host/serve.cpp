@@ -179,7 +179,8 @@ struct Server::Impl {
                 if (!ExactPolynomial::is_squarefree(opt.coefficients)) {
                     throw std::invalid_argument("polynomial is not squarefree");
                 }
-                a.reset(new Active(std::move(request)));
+                // A copy: `request` must stay whole for fail_now if this throws.
+                a.reset(new Active(request));
             } catch (const std::exception &e) {
                 fail_now(std::move(request), next.second, e.what());
                 continue;
