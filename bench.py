@@ -29,8 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 BATCH = 65536
 BOUND = 5_000_000
 LAST = 5_000_000
-# MI355X LDS read bandwidth: 256 CUs x 128 B/clk x 2.4 GHz.
-LDS_PEAK_GBS = 256 * 128 * 2.4
+# MI355X LDS read bandwidth: 256 CUs x 256 B/clk x 2.4 GHz (CDNA4 doubles the
+# CDNA3 128 B/clk; ratpoints-gpu_amd/bin/lds_roof measures 134 B/clk for
+# 8-byte and 192 B/clk for aligned 16-byte reads per lane, above the old
+# figure).
+LDS_PEAK_GBS = 256 * 256 * 2.4
 # Algorithmic LDS bytes per site, the reference's fast-stage work: 14 mask
 # words of 4 bytes per 32 numerators (src/sieve.cu:296-331) = 1.75 B/site.
 ALGORITHMIC_LDS_BYTES_PER_SITE = 14 * 4 / 32

@@ -3,7 +3,10 @@
 // Every wave reads 64 consecutive 8-byte words (lane l reads word s + l, s
 // wave-uniform) -- the sieve kernel's access pattern -- and XORs them into a
 // register; the loop offsets vary every iteration so nothing is hoisted.
-// Prints the achieved LDS read bandwidth for 8- and 16-byte reads per lane.
+// Prints the achieved LDS read bandwidth for: 8-byte reads per lane; 16-byte
+// reads per lane at an 8-byte-aligned base and at a 16-byte-aligned base;
+// and pairs of 8-byte reads 64 words apart (the dense stage's two words per
+// lane, one ds_read2st64_b64).
 // `lds_roof copy` instead runs one plain 256 MiB device-to-device copy kernel,
 // the reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters.
 //
@@ -20,27 +23,40 @@ namespace {
 constexpr int kWords = 8192;  // 64 KiB of LDS
 constexpr int kThreads = 512;
 
-template <int VEC>
+// VEC words per lane, contiguous (STRIDE 1) or 64 words apart (STRIDE 64);
+// ALIGN 2 rounds the wave's base down to an even word (16-byte aligned).
+template <int VEC, int STRIDE = 1, int ALIGN = 1>
 __global__ __launch_bounds__(kThreads) void lds_read_kernel(
     int iterations, unsigned long long *out) {
-    __shared__ uint64_t buf[kWords + 64 * VEC];
-    for (int i = threadIdx.x; i < kWords + 64 * VEC; i += kThreads) {
+    constexpr int kPad = 64 * VEC * STRIDE;
+    __shared__ __attribute__((aligned(16))) uint64_t buf[kWords + kPad];
+    for (int i = threadIdx.x; i < kWords + kPad; i += kThreads) {
         buf[i] = 0x9E3779B97F4A7C15ULL * (i + 1);
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
     uint32_t s = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) * 977u);
-    uint64_t acc[VEC];
+    // Each loaded 64-bit word is folded into one 32-bit accumulator with a
+    // single three-input XOR (v_bitop3_b32), so the VALU work per byte read
+    // stays well below the LDS rate being measured.
+    uint32_t acc[VEC];
     for (int v = 0; v < VEC; ++v) {
         acc[v] = 0;
     }
     for (int it = 0; it < iterations; ++it) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const uint64_t *p = buf + ((s + 131u * k) & (kWords - 1)) + lane * VEC;
+            const uint32_t base = ((s + 131u * k) & (kWords - 1)) & ~(ALIGN - 1u);
+            const int step = STRIDE == 1 ? VEC : 1;
+            const uint64_t *p = buf + base + lane * step;
+            if (ALIGN == 2) {
+                p = static_cast<const uint64_t *>(__builtin_assume_aligned(p, 16));
+            }
 #pragma unroll
             for (int v = 0; v < VEC; ++v) {
-                acc[v] ^= p[v];
+                const uint64_t w = p[v * STRIDE];
+                acc[v] = __builtin_amdgcn_bitop3_b32(acc[v], static_cast<uint32_t>(w),
+                                                     static_cast<uint32_t>(w >> 32), 0x96);
             }
         }
         s += 1031u;
@@ -54,15 +70,17 @@ __global__ __launch_bounds__(kThreads) void lds_read_kernel(
     }
 }
 
-template <int VEC>
+template <int VEC, int STRIDE = 1, int ALIGN = 1>
 double measure(int iterations, int blocks, unsigned long long *d_out) {
     hipEvent_t a, b;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
-    hipLaunchKernelGGL(lds_read_kernel<VEC>, dim3(blocks), dim3(kThreads), 0, 0, 10, d_out);
+    hipLaunchKernelGGL((lds_read_kernel<VEC, STRIDE, ALIGN>), dim3(blocks), dim3(kThreads), 0, 0,
+                       10, d_out);
     (void)hipDeviceSynchronize();
     (void)hipEventRecord(a);
-    hipLaunchKernelGGL(lds_read_kernel<VEC>, dim3(blocks), dim3(kThreads), 0, 0, iterations, d_out);
+    hipLaunchKernelGGL((lds_read_kernel<VEC, STRIDE, ALIGN>), dim3(blocks), dim3(kThreads), 0, 0,
+                       iterations, d_out);
     (void)hipEventRecord(b);
     (void)hipEventSynchronize(b);
     float ms = 0;
@@ -118,12 +136,16 @@ int main(int argc, char **argv) {
     }
     const double b64 = measure<1>(iterations, blocks, d_out);
     const double b128 = measure<2>(iterations, blocks, d_out);
+    const double b128a = measure<2, 1, 2>(iterations, blocks, d_out);
+    const double st64 = measure<2, 64>(iterations, blocks, d_out);
+    const double per_clk = 1e9 / (prop.multiProcessorCount * prop.clockRate * 1e3);
     std::printf("device=%s cus=%d clock_khz=%d\n", prop.name, prop.multiProcessorCount,
                 prop.clockRate);
-    std::printf("lds_read_b64_GBs=%.1f lds_read_b128_GBs=%.1f\n", b64, b128);
-    std::printf("bytes_per_clk_per_cu_b64=%.1f bytes_per_clk_per_cu_b128=%.1f\n",
-                b64 * 1e9 / (prop.multiProcessorCount * prop.clockRate * 1e3),
-                b128 * 1e9 / (prop.multiProcessorCount * prop.clockRate * 1e3));
+    std::printf("lds_read_b64_GBs=%.1f lds_read_b128_GBs=%.1f lds_read_b128_aligned_GBs=%.1f "
+                "lds_read2st64_b64_GBs=%.1f\n", b64, b128, b128a, st64);
+    std::printf("bytes_per_clk_per_cu_b64=%.1f bytes_per_clk_per_cu_b128=%.1f "
+                "bytes_per_clk_per_cu_b128_aligned=%.1f bytes_per_clk_per_cu_read2st64=%.1f\n",
+                b64 * per_clk, b128 * per_clk, b128a * per_clk, st64 * per_clk);
     (void)hipFree(d_out);
     return 0;
 }

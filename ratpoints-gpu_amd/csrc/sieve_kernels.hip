@@ -212,23 +212,37 @@ __device__ __forceinline__ bool emit_survivors(const OutDesc &O, long long bound
 //   FAST   selected primes ANDed densely for every word; the other
 //          32 - FAST are applied only to words with bits left
 //   WAVES  waves per workgroup; each wave sweeps a contiguous word range
+//   WIDE   (U = 2 only) rows of 16-byte elements: element e holds entries e
+//          and e + 64, the two words a lane takes per iteration, so the
+//          dense stage is one aligned ds_read_b128 per row.  gfx950 serves
+//          16-byte LDS reads at about 1.6x the bytes per clock of paired
+//          8-byte reads; the rows take twice the LDS, so such a tile runs
+//          one workgroup per CU.
 // ---------------------------------------------------------------------------
-template <int U, int FAST, int WAVES>
+using u64x2 = unsigned long long __attribute__((ext_vector_type(2)));
+
+template <int U, int FAST, int WAVES, bool WIDE = false>
 struct RowTile {
     using Base = RowTile;
     static constexpr int kWaves = WAVES;
     static constexpr int kThreads = WAVES * 64;
     static constexpr int kStride = 64 * U;
-    static constexpr int kSlot = kMaxPrime + 64 * U;  // >= p + 64 U - 1 words
+    // Row elements per slot: >= p + 64 U - 1 entries, or p + 63 wide
+    // elements; kSlotWords 64-bit words.
+    static constexpr int kSlot = WIDE ? kMaxPrime + 64 : kMaxPrime + 64 * U;
+    static constexpr int kSlotWords = WIDE ? 2 * kSlot : kSlot;
+    // Entries computed per row: a wide element also holds entry e + 64.
+    static constexpr int kEntries = WIDE ? kSlot + 64 : kSlot;
     static constexpr int kLate = kSelected - FAST;
     // LDS layout, in 64-bit words: rows, residue strings, then six arrays of
     // 32 uint32 (p, (-A) mod p, b mod p, candidate index, ts mod p, magic).
-    static constexpr int kCext = FAST * kSlot;
+    static constexpr int kCext = FAST * kSlotWords;
     static constexpr int kSmall = kCext + kSelected * 10;
     static constexpr int kLdsWords = kSmall + 3 * kSelected;
     static_assert(kStride < 293, "a row step must stay below every prime");
     static_assert(FAST >= 1 && FAST <= kSelected, "dense prime count");
     static_assert(kLate <= 32, "late primes must fit a half wave");
+    static_assert(!WIDE || U == 2, "wide rows pair the two words of U = 2");
 
     // Word range of wave `wave` in [ts, te), in whole strides.
     __device__ static void wave_range(uint64_t ts, uint64_t te, int wave,
@@ -312,15 +326,17 @@ struct RowTile {
         // reads, so they are written without a test.  All string reads of a
         // thread are independent and can be in flight together; the window
         // at bit o is two funnel shifts of three 32-bit string words.
-        static_assert(kSlot % 64 == 0 && kThreads % 64 == 0, "wave-uniform slot tests");
+        static_assert(kSlot % 64 == 0 && kEntries % 64 == 0 && kThreads % 64 == 0,
+                      "wave-uniform slot tests");
+        static_assert(kEntries <= 2 * kThreads, "two entries per thread at most");
         // Offsets first, then every string read, then the funnel shifts and
         // the row stores, so a thread's reads are all in flight together and
         // none waits behind a store to the same LDS array.
         const uint32_t e0 = static_cast<uint32_t>(tid);
-        if (e0 < static_cast<uint32_t>(kSlot)) {
+        if (e0 < static_cast<uint32_t>(kEntries)) {
             build_entries(pool, cext, P, C, M, e0);
         }
-        if (kThreads < kSlot && e0 < static_cast<uint32_t>(kSlot - kThreads)) {
+        if (kThreads < kEntries && e0 < static_cast<uint32_t>(kEntries - kThreads)) {
             build_entries(pool, cext, P, C, M, e0 + kThreads);
         }
     }
@@ -346,7 +362,19 @@ struct RowTile {
         for (int s = 0; s < FAST; ++s) {
             const uint32_t lo = __builtin_amdgcn_alignbit(d1[s], d0[s], o[s] & 31);
             const uint32_t hi = __builtin_amdgcn_alignbit(d2[s], d1[s], o[s] & 31);
-            pool[s * kSlot + e] = (static_cast<uint64_t>(hi) << 32) | lo;
+            const uint64_t w = (static_cast<uint64_t>(hi) << 32) | lo;
+            if (WIDE) {
+                // Entry e is the low half of element e and the high half of
+                // element e - 64 (both tests are wave-uniform).
+                if (e < static_cast<uint32_t>(kSlot)) {
+                    pool[s * kSlotWords + 2 * e] = w;
+                }
+                if (e >= 64u) {
+                    pool[s * kSlotWords + 2 * (e - 64) + 1] = w;
+                }
+            } else {
+                pool[s * kSlotWords + e] = w;
+            }
         }
     }
 
@@ -361,10 +389,16 @@ struct RowTile {
         uint64_t x[FAST][U];
 #pragma unroll
         for (int s = 0; s < FAST; ++s) {
-            const uint64_t *r = pool + s * kSlot + off[s] + lane;
+            if (WIDE) {
+                const u64x2 v = reinterpret_cast<const u64x2 *>(pool + s * kSlotWords)[off[s] + lane];
+                x[s][0] = v.x;
+                x[s][U - 1] = v.y;
+            } else {
+                const uint64_t *r = pool + s * kSlotWords + off[s] + lane;
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                x[s][u] = r[64 * u];
+                for (int u = 0; u < U; ++u) {
+                    x[s][u] = r[64 * u];
+                }
             }
         }
         uint32_t lo[U], hi[U];
@@ -533,13 +567,14 @@ struct RowTile {
 // reservation.  The resume cursor is W + u (iteration W, sub-word u), the
 // oldest uncommitted queue entry, so nothing is emitted twice.
 // ---------------------------------------------------------------------------
-template <int U, int FAST, int WAVES>
+template <int U, int FAST, int WAVES, bool WIDE = false>
 struct RowTileQ {
-    using Base = RowTile<U, FAST, WAVES>;
+    using Base = RowTile<U, FAST, WAVES, WIDE>;
     static constexpr int kWaves = WAVES;
     static constexpr int kStride = Base::kStride;
     static constexpr int kSlot = Base::kSlot;
     static constexpr int kQueue = 128;
+    static constexpr int kGroup = 3;  // late primes per drain step
     // Per wave: 128 masks (128 words) and 128 uint32 word offsets (64 words).
     static constexpr int kQueueWords = kQueue + kQueue / 2;
     static constexpr int kQueueBase = Base::kLdsWords;
@@ -581,26 +616,44 @@ struct RowTileQ {
             rel[k] = valid ? qw[idx] << 6 : 0u;
         }
         wave_sync();
-        for (int s = FAST; s < kSelected; ++s) {
+        // Late primes in groups of kGroup: the group's string reads are all
+        // in flight together, and the wave-uniform exit is tested once per
+        // group (a drain typically ends within a few primes, so one LDS
+        // round trip per group instead of per prime).
+        for (int s0 = FAST; s0 < kSelected; s0 += kGroup) {
             if (!__ballot((m[0] | m[1]) != 0)) {
                 break;
             }
-            const uint32_t p = __builtin_amdgcn_readlane(lp, s - FAST);
-            const uint32_t magic = __builtin_amdgcn_readlane(lmagic, s - FAST);
-            const uint32_t cb = __builtin_amdgcn_readlane(lcb, s - FAST);
-            const uint32_t *cx = cext + s * 20;
+            uint64_t win[kGroup][2];
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                if (m[k]) {
-                    const uint32_t o = mod_small(cb + rel[k], p, magic);
-                    const uint32_t *d = cx + (o >> 5);
-                    const uint32_t d0 = d[0];
-                    const uint32_t d1 = d[1];
-                    const uint32_t d2 = d[2];
-                    const uint32_t lo = __builtin_amdgcn_alignbit(d1, d0, o & 31);
-                    const uint32_t hi = __builtin_amdgcn_alignbit(d2, d1, o & 31);
-                    m[k] &= (static_cast<uint64_t>(hi) << 32) | lo;
+            for (int j = 0; j < kGroup; ++j) {
+                const int s = s0 + j;
+                win[j][0] = ~0ULL;
+                win[j][1] = ~0ULL;
+                if (s < kSelected) {
+                    const uint32_t p = __builtin_amdgcn_readlane(lp, s - FAST);
+                    const uint32_t magic = __builtin_amdgcn_readlane(lmagic, s - FAST);
+                    const uint32_t cb = __builtin_amdgcn_readlane(lcb, s - FAST);
+                    const uint32_t *cx = cext + s * 20;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        if (m[k]) {
+                            const uint32_t o = mod_small(cb + rel[k], p, magic);
+                            const uint32_t *d = cx + (o >> 5);
+                            const uint32_t d0 = d[0];
+                            const uint32_t d1 = d[1];
+                            const uint32_t d2 = d[2];
+                            const uint32_t lo = __builtin_amdgcn_alignbit(d1, d0, o & 31);
+                            const uint32_t hi = __builtin_amdgcn_alignbit(d2, d1, o & 31);
+                            win[j][k] = (static_cast<uint64_t>(hi) << 32) | lo;
+                        }
+                    }
                 }
+            }
+#pragma unroll
+            for (int j = 0; j < kGroup; ++j) {
+                m[0] &= win[j][0];
+                m[1] &= win[j][1];
             }
         }
         if (!__ballot((m[0] | m[1]) != 0)) {
@@ -894,11 +947,13 @@ struct RowRun {
         __syncthreads();  // the previous line's sweep is over
         flag_value = flag ? static_cast<int>(uniform(static_cast<uint32_t>(*flag))) : 0;
         store(pool, v, tid);
+        // Two waves select in parallel: this line's primes (wave 0) and the
+        // next line's candidate indices (the last wave).
         if (wave == 0) {
             select(pool, b, ts, lane);
-            if (more) {
-                select_next(pool, bn, lane);
-            }
+        }
+        if (wave == Row::kWaves - 1 && more) {
+            select_next(pool, bn, lane);
         }
         __syncthreads();
         Base::build_rows(pool, tid);
@@ -1177,7 +1232,7 @@ template <class Tile, int WAVES>
 __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4))) void sieve_kernel(
     const SieveParams P) {
     using Run = RowRun<Tile>;
-    __shared__ uint64_t pool[Run::kLdsWords];
+    __shared__ __attribute__((aligned(16))) uint64_t pool[Run::kLdsWords];
     __shared__ unsigned s_first;
     const LineDesc &L = P.line;
     const int tid = threadIdx.x;
@@ -1248,7 +1303,7 @@ template <int WAVES, int NB>
 __global__ __launch_bounds__(WAVES * 64) void direct_kernel(const SieveParams P) {
     using Run = DirectRun<WAVES, NB>;
     using Tile = DirectTile<WAVES, NB>;
-    __shared__ uint64_t pool[Run::kLdsWords];
+    __shared__ __attribute__((aligned(16))) uint64_t pool[Run::kLdsWords];
     __shared__ unsigned s_first;
     const LineDesc &L = P.line;
     const int tid = threadIdx.x;
@@ -1325,7 +1380,7 @@ __global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4)))
     using DRun = DirectRun<WAVES, kDirectMaxDenominators>;
     constexpr int kWords = Run::kLdsWords > DRun::kLdsWords ? Run::kLdsWords
                                                              : DRun::kLdsWords;
-    __shared__ uint64_t pool[kWords];
+    __shared__ __attribute__((aligned(16))) uint64_t pool[kWords];
     __shared__ unsigned long long s_tile;
     __shared__ int s_curve;
     __shared__ int s_skip;
@@ -1480,6 +1535,11 @@ hipError_t multi_occupancy(int *blocks) {
      launch_multi_variant<RowTileQ<U, F, W>, W>,                           \
      multi_occupancy<RowTileQ<U, F, W>, W>}
 
+#define RPS_VARIANT_W(F, W)                                                \
+    {"w_u2_f" #F "_w" #W, 2, F, W,                                         \
+     launch_variant<RowTileQ<2, F, W, true>, W>,                           \
+     launch_multi_variant<RowTileQ<2, F, W, true>, W>,                     \
+     multi_occupancy<RowTileQ<2, F, W, true>, W>}
 const VariantInfo kVariants[] = {
     RPS_VARIANT_Q(2, 12, 8),
     RPS_VARIANT_Q(2, 11, 8),
@@ -1487,10 +1547,16 @@ const VariantInfo kVariants[] = {
     RPS_VARIANT_Q(1, 12, 8),
     RPS_VARIANT_Q(2, 12, 16),
     RPS_VARIANT(2, 14, 8),
+    RPS_VARIANT_W(12, 16),
+    RPS_VARIANT_W(13, 16),
+    RPS_VARIANT_W(14, 16),
+    RPS_VARIANT_W(12, 12),
+    RPS_VARIANT_W(12, 8),
 };
 
 #undef RPS_VARIANT
 #undef RPS_VARIANT_Q
+#undef RPS_VARIANT_W
 
 }  // namespace
 

@@ -44,7 +44,7 @@ def test_variants_listed(pkg):
     names = pkg.variants()
     assert names and names[0] == "q_u2_f12_w8"
     for n in names:
-        assert re.fullmatch(r"(q_)?u\d_f\d+_w\d+", n)
+        assert re.fullmatch(r"([qw]_)?u\d_f\d+_w\d+", n)
 
 
 def test_tuning_table_names_compiled_variants(pkg):
