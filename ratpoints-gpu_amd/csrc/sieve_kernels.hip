@@ -221,22 +221,61 @@ __device__ __forceinline__ bool emit_survivors(const OutDesc &O, long long bound
 // ---------------------------------------------------------------------------
 using u64x2 = unsigned long long __attribute__((ext_vector_type(2)));
 
+// The s-th largest prime below 512 (s = 0: 509): the largest prime a dense
+// row s can hold, since the selection is the candidates in descending order.
+__host__ __device__ constexpr int largest_prime_below_512(int s) {
+    constexpr int kDesc[kCandidates] = {
+        509, 503, 499, 491, 487, 479, 467, 463, 461, 457, 449, 443,
+        439, 433, 431, 421, 419, 409, 401, 397, 389, 383, 379, 373,
+        367, 359, 353, 349, 347, 337, 331, 317, 313, 311, 307, 293};
+    return kDesc[s];
+}
+constexpr bool is_prime_below_512(int n) {
+    for (int d = 2; d * d <= n; ++d) {
+        if (n % d == 0) {
+            return false;
+        }
+    }
+    return n > 1;
+}
+constexpr bool descending_primes_ok() {
+    int s = 0;
+    for (int n = 511; n > 1 && s < kCandidates; --n) {
+        if (is_prime_below_512(n) && largest_prime_below_512(s++) != n) {
+            return false;
+        }
+    }
+    return s == kCandidates;
+}
+static_assert(descending_primes_ok(), "the 36 largest primes below 512, descending");
+
 template <int U, int FAST, int WAVES, bool WIDE = false>
 struct RowTile {
     using Base = RowTile;
     static constexpr int kWaves = WAVES;
     static constexpr int kThreads = WAVES * 64;
     static constexpr int kStride = 64 * U;
-    // Row elements per slot: >= p + 64 U - 1 entries, or p + 63 wide
-    // elements; kSlotWords 64-bit words.
+    // Row elements per slot: >= p + 64 U - 1 entries (kSlot, every row), or
+    // p + 63 wide elements, sized per row by the largest prime it can hold.
     static constexpr int kSlot = WIDE ? kMaxPrime + 64 : kMaxPrime + 64 * U;
-    static constexpr int kSlotWords = WIDE ? 2 * kSlot : kSlot;
-    // Entries computed per row: a wide element also holds entry e + 64.
-    static constexpr int kEntries = WIDE ? kSlot + 64 : kSlot;
+    __host__ __device__ static constexpr int slot_elems(int s) {
+        return WIDE ? largest_prime_below_512(s) + 64 : kSlot;
+    }
+    // First 64-bit word of row s.
+    __host__ __device__ static constexpr int row_base(int s) {
+        int w = 0;
+        for (int j = 0; j < s; ++j) {
+            w += (WIDE ? 2 : 1) * slot_elems(j);
+        }
+        return w;
+    }
+    // Entries computed per row (a wide element also holds entry e + 64),
+    // rounded up to whole waves.
+    static constexpr int kEntries = WIDE ? (slot_elems(0) + 64 + 63) / 64 * 64 : kSlot;
     static constexpr int kLate = kSelected - FAST;
     // LDS layout, in 64-bit words: rows, residue strings, then six arrays of
     // 32 uint32 (p, (-A) mod p, b mod p, candidate index, ts mod p, magic).
-    static constexpr int kCext = FAST * kSlotWords;
+    static constexpr int kCext = row_base(FAST);
     static constexpr int kSmall = kCext + kSelected * 10;
     static constexpr int kLdsWords = kSmall + 3 * kSelected;
     static_assert(kStride < 293, "a row step must stay below every prime");
@@ -365,15 +404,16 @@ struct RowTile {
             const uint64_t w = (static_cast<uint64_t>(hi) << 32) | lo;
             if (WIDE) {
                 // Entry e is the low half of element e and the high half of
-                // element e - 64 (both tests are wave-uniform).
-                if (e < static_cast<uint32_t>(kSlot)) {
-                    pool[s * kSlotWords + 2 * e] = w;
+                // element e - 64, for the elements row s has.
+                const uint32_t n = static_cast<uint32_t>(slot_elems(s));
+                if (e < n) {
+                    pool[row_base(s) + 2 * e] = w;
                 }
-                if (e >= 64u) {
-                    pool[s * kSlotWords + 2 * (e - 64) + 1] = w;
+                if (e >= 64u && e < n + 64u) {
+                    pool[row_base(s) + 2 * (e - 64) + 1] = w;
                 }
             } else {
-                pool[s * kSlotWords + e] = w;
+                pool[row_base(s) + e] = w;
             }
         }
     }
@@ -390,11 +430,11 @@ struct RowTile {
 #pragma unroll
         for (int s = 0; s < FAST; ++s) {
             if (WIDE) {
-                const u64x2 v = reinterpret_cast<const u64x2 *>(pool + s * kSlotWords)[off[s] + lane];
+                const u64x2 v = reinterpret_cast<const u64x2 *>(pool + row_base(s))[off[s] + lane];
                 x[s][0] = v.x;
                 x[s][U - 1] = v.y;
             } else {
-                const uint64_t *r = pool + s * kSlotWords + off[s] + lane;
+                const uint64_t *r = pool + row_base(s) + off[s] + lane;
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     x[s][u] = r[64 * u];
@@ -713,31 +753,34 @@ struct RowTileQ {
         uint32_t n_clean = clean_end > ws
             ? static_cast<uint32_t>((clean_end - ws) / kStride) : 0u;
         n_clean = n_clean < n_iter ? n_clean : n_iter;
-        Queue q{qm, qw, 0u, ws};
+        Queue q{qm, qw, 0u};
         const Late late{lp, lmagic, lcb};
+        // Iteration it starts at word ws + it kStride, rel0 + it kStride from ts.
+        const uint32_t rel0 = static_cast<uint32_t>(ws - ts);
         uint32_t it = 0;
         if (skip0 && n_iter) {
-            if (!step<true>(pool, ws, we, last_word, skip0, off, prime, q, late, b, ts, L, O,
-                            request, lane)) {
-                return q.oldest;
+            if (!step<true>(pool, ws, rel0, we, last_word, skip0, off, prime, q, late, b, ts, L,
+                            O, request, lane)) {
+                return oldest(q, ts);
             }
             it = 1;
         }
         for (; it < n_clean; ++it) {
-            if (!step<false>(pool, ws + static_cast<uint64_t>(it) * kStride, we, last_word, 0u,
-                             off, prime, q, late, b, ts, L, O, request, lane)) {
-                return q.oldest;
+            if (!step<false>(pool, 0, rel0 + it * kStride, we, last_word, 0u, off, prime, q,
+                             late, b, ts, L, O, request, lane)) {
+                return oldest(q, ts);
             }
         }
         for (; it < n_iter; ++it) {
-            if (!step<true>(pool, ws + static_cast<uint64_t>(it) * kStride, we, last_word, 0u,
-                            off, prime, q, late, b, ts, L, O, request, lane)) {
-                return q.oldest;
+            if (!step<true>(pool, ws + static_cast<uint64_t>(it) * kStride, rel0 + it * kStride,
+                            we, last_word, 0u, off, prime, q, late, b, ts, L, O, request,
+                            lane)) {
+                return oldest(q, ts);
             }
         }
         if (q.queued && !drain(pool, q.qm, q.qw, late.p, late.magic, late.cb, q.queued, b, ts,
                                L, O, request, lane)) {
-            return q.oldest;
+            return oldest(q, ts);
         }
         return we;
     }
@@ -746,8 +789,16 @@ struct RowTileQ {
         uint64_t *qm;
         uint32_t *qw;
         uint32_t queued;
-        uint64_t oldest;  // cursor (W + u) of the oldest queued entry
     };
+
+    // Resume cursor (W + u) of the oldest queued entry, from its word offset
+    // r = (W - ts) + lane + 64 u: iterations start at multiples of kStride
+    // from ts, and lane + 64 u < kStride.
+    __device__ static uint64_t oldest(const Queue &q, uint64_t ts) {
+        wave_sync();
+        const uint32_t r = uniform(q.qw[0]);
+        return ts + (r & ~static_cast<uint32_t>(kStride - 1)) + ((r & (kStride - 1)) >> 6);
+    }
 
     struct Late {
         uint32_t p, magic, cb;  // lane j: late prime FAST + j
@@ -757,9 +808,10 @@ struct RowTileQ {
     // line's last word and the first `skip` sub-words), queue pushes and,
     // when 64 or more entries wait, a drain.  False when a drain's survivors
     // did not fit the buffer.
+    // W is read only by the EDGE body; rel_w = W - ts.
     template <bool EDGE>
-    __device__ static bool step(uint64_t *pool, uint64_t W, uint64_t we, uint64_t last_word,
-                                uint32_t skip, uint32_t (&off)[FAST],
+    __device__ static bool step(uint64_t *pool, uint64_t W, uint32_t rel_w, uint64_t we,
+                                uint64_t last_word, uint32_t skip, uint32_t (&off)[FAST],
                                 const uint32_t (&prime)[FAST], Queue &q, const Late &late,
                                 int b, uint64_t ts, const LineDesc &L, const OutDesc &O,
                                 int request, int lane) {
@@ -784,7 +836,7 @@ struct RowTileQ {
         if (!__ballot(any != 0)) {
             return true;
         }
-        const uint32_t rel = static_cast<uint32_t>(W - ts) + lane;
+        const uint32_t rel = rel_w + lane;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint64_t bal = __ballot(m[u] != 0);
@@ -794,9 +846,6 @@ struct RowTileQ {
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
                 static_cast<uint32_t>(bal >> 32),
                 __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bal), 0u));
-            if (q.queued == 0) {
-                q.oldest = W + u;
-            }
             if (m[u]) {
                 q.qm[q.queued + rank] = m[u];
                 q.qw[q.queued + rank] = rel + 64u * u;
@@ -1550,8 +1599,8 @@ const VariantInfo kVariants[] = {
     RPS_VARIANT_W(12, 16),
     RPS_VARIANT_W(13, 16),
     RPS_VARIANT_W(14, 16),
-    RPS_VARIANT_W(12, 12),
-    RPS_VARIANT_W(12, 8),
+    RPS_VARIANT_W(15, 16),
+    RPS_VARIANT_W(16, 12),
 };
 
 #undef RPS_VARIANT
