@@ -145,7 +145,9 @@ typedef int (*rps_multi_sink)(void *user, const int32_t *request,
  * and modes) in one persistent launch: workgroups pull (curve, denominator
  * block) and (curve, line chunk) tiles of all requests from a device work
  * counter, so short searches share the GPU instead of each paying a launch
- * and a synchronisation.  Each request's survivor set equals what rps_sieve
+ * and a synchronisation.  With the variant left to the tuning table,
+ * requests whose lines take wide-row tiles (one workgroup per CU) get a
+ * second launch of their own.  Each request's survivor set equals what rps_sieve
  * gives for it alone.  A request whose survivors overflow the buffer during
  * the shared launch is searched again alone (stats->requeued counts them).
  * No counterpart in the reference, which searches one curve per process

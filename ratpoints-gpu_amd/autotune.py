@@ -22,7 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "csrc", "tuning_table.inc")
 # Upper ends of the buckets (words per line) and the length timed for each.
 BUCKETS = [(1536, 1024), (3072, 2048), (6144, 4096), (12288, 8192),
-           (24576, 16384), (49152, 32768), (1 << 62, 156251)]
+           (24576, 16384), (49152, 32768), (98304, 65536), (1 << 62, 156251)]
 
 
 def load(name, path):

@@ -390,16 +390,17 @@ void row_tiling(rps::LineDesc &L, unsigned long long chunks, int stride) {
     L.tile_t = 1;
 }
 
-// Whole-line row tiles of lines shorter than kRunWords take several
+// Whole-line row tiles of lines shorter than run_words take several
 // consecutive denominators (a run) so that per-tile work stays near
-// kRunWords words; at most max_run lines per tile.
-constexpr unsigned long long kRunWords = 1ULL << 17;
-
-void row_runs(rps::LineDesc &L, long long max_run) {
-    if (L.tiles_per_t != 1 || L.words >= kRunWords) {
+// run_words words (the variant's choice: one workgroup per CU wants longer
+// runs, whose next-line prefetch hides the string loads); at most max_run
+// lines per tile.
+void row_runs(rps::LineDesc &L, long long max_run, int run_log2) {
+    const unsigned long long run_words = 1ULL << run_log2;
+    if (L.tiles_per_t != 1 || L.words >= run_words) {
         return;
     }
-    const long long want = static_cast<long long>(kRunWords / L.words);
+    const long long want = static_cast<long long>(run_words / L.words);
     L.tile_t = static_cast<int>(std::max(1LL, std::min(want, max_run)));
 }
 
@@ -759,7 +760,8 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
                 n = std::min<long long>(n, static_cast<long long>(kMaxTilesPerLaunch / chunks));
                 chunks = line_chunks(ctx->min_tiles, n, words, stride, var->waves);
                 row_tiling(bt.line, chunks, stride);
-                row_runs(bt.line, std::max(1LL, n / std::max(1LL, ctx->min_tiles)));
+                row_runs(bt.line, std::max(1LL, n / std::max(1LL, ctx->min_tiles)),
+                         var->run_log2);
             }
             bt.line.first_t = static_cast<int>(next_t);
             bt.line.n_t = static_cast<int>(n);
@@ -875,11 +877,14 @@ int rps_check_request(const rps_request *request, char *message, size_t message_
     return rc;
 }
 
-int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
-                   rps_multi_sink sink, void *user, rps_stats *stats) {
-    if (!ctx) {
-        return RPS_EINVAL;
-    }
+}  // extern "C"
+
+namespace {
+
+// One persistent launch over all the requests, with `forced` (>= 0) or the
+// tuning table's variant for the request with the most sites.
+int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
+                      rps_multi_sink sink, void *user, rps_stats *stats, int forced) {
     rps_stats local{};
     rps_stats &st = stats ? *stats : local;
     st = rps_stats{};
@@ -907,7 +912,7 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
         RPS_HIP(hipSetDevice(ctx->device));
         // One variant per launch: the forced one, or the tuning table's
         // choice for the request with the most sites.
-        int chosen = ctx->variant;
+        int chosen = forced >= 0 ? forced : ctx->variant;
         if (chosen < 0) {
             int largest = 0;
             double most = -1.0;
@@ -941,7 +946,7 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
                 const unsigned long long chunks = std::max(
                     (L.words + (1ULL << 18) - 1) >> 18, 1ULL);
                 row_tiling(L, chunks, stride);
-                row_runs(L, L.n_t);
+                row_runs(L, L.n_t, var->run_log2);
             }
             lines[r] = L;
             const double line_sites = 2.0 * static_cast<double>(L.bound) + 1.0;
@@ -1263,6 +1268,106 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
         ctx->error = "out of host memory";
         return RPS_ENOMEM;
     }
+}
+
+// Survivor sink of one group of requests: maps the group's request indices
+// back to the caller's.
+struct GroupSink {
+    rps_multi_sink sink;
+    void *user;
+    const std::vector<int32_t> *index;
+    std::vector<int32_t> mapped;
+
+    static int call(void *u, const int32_t *req, const int64_t *num, const int32_t *den,
+                    size_t count) {
+        GroupSink &g = *static_cast<GroupSink *>(u);
+        g.mapped.resize(count);
+        for (size_t i = 0; i < count; ++i) {
+            g.mapped[i] = (*g.index)[static_cast<size_t>(req[i])];
+        }
+        return g.sink(g.user, g.mapped.data(), num, den, count);
+    }
+};
+
+void add_stats(rps_stats &into, const rps_stats &s) {
+    into.survivor_count += s.survivor_count;
+    into.basis_ms += s.basis_ms;
+    into.sieve_ms += s.sieve_ms;
+    into.plan_ms += s.plan_ms;
+    into.workspace_ms += s.workspace_ms;
+    into.basis_wall_ms += s.basis_wall_ms;
+    into.survivor_setup_ms += s.survivor_setup_ms;
+    into.verification_ms += s.verification_ms;
+    into.sieve_wall_ms += s.sieve_wall_ms;
+    into.launches += s.launches;
+    into.batches += s.batches;
+    into.requeued += s.requeued;
+}
+
+}  // namespace
+
+extern "C" {
+
+// With the variant left to the tuning table, requests whose lines want a
+// one-workgroup-per-CU (wide-row) tile run in a launch of their own, and
+// the others share a launch of the tuning table's choice for the largest of
+// them: a launch has one LDS layout, and the wide one would cost the short
+// lines half their occupancy.
+int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
+                   rps_multi_sink sink, void *user, rps_stats *stats) {
+    if (!ctx) {
+        return RPS_EINVAL;
+    }
+    if (ctx->variant >= 0 || n_requests < 2 || !requests || !sink
+        || n_requests > (size_t(1) << 20)) {
+        return sieve_many_launch(ctx, requests, n_requests, sink, user, stats, -1);
+    }
+    std::vector<int32_t> groups[2];  // [0] two-workgroup tiles, [1] wide
+    try {
+        for (size_t r = 0; r < n_requests; ++r) {
+            validate(&requests[r]);
+            const int v = tuned_variant(line_geometry(&requests[r]).words);
+            groups[rps::variant(v)->wide ? 1 : 0].push_back(static_cast<int32_t>(r));
+        }
+    } catch (const ApiError &e) {
+        ctx->last_kernel_ms = 0.0;
+        ctx->last_launches = 0;
+        ctx->last_sites = 0.0;
+        if (stats) {
+            *stats = rps_stats{};
+        }
+        return fail(ctx, e);
+    }
+    if (groups[0].empty() || groups[1].empty()) {
+        return sieve_many_launch(ctx, requests, n_requests, sink, user, stats, -1);
+    }
+    rps_stats total{};
+    double kernel_ms = 0.0, sites = 0.0;
+    uint64_t launches = 0;
+    int rc = RPS_OK;
+    for (const std::vector<int32_t> &g : groups) {
+        std::vector<rps_request> sub(g.size());
+        for (size_t k = 0; k < g.size(); ++k) {
+            sub[k] = requests[g[k]];
+        }
+        GroupSink gs{sink, user, &g, {}};
+        rps_stats one{};
+        rc = sieve_many_launch(ctx, sub.data(), sub.size(), GroupSink::call, &gs, &one, -1);
+        add_stats(total, one);
+        kernel_ms += ctx->last_kernel_ms;
+        launches += ctx->last_launches;
+        sites += ctx->last_sites;
+        if (rc != RPS_OK) {
+            break;
+        }
+    }
+    ctx->last_kernel_ms = kernel_ms;
+    ctx->last_launches = launches;
+    ctx->last_sites = sites;
+    if (stats) {
+        *stats = total;
+    }
+    return rc;
 }
 
 }  // extern "C"

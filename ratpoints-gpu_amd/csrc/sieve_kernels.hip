@@ -1576,16 +1576,16 @@ hipError_t multi_occupancy(int *blocks) {
 // "u2_f14_w8": RowTile with U = 2, FAST = 14, 8 waves; "q_u2_f12_w8": the
 // same shape with the compaction-queue late stage.
 #define RPS_VARIANT(U, F, W)                                               \
-    {"u" #U "_f" #F "_w" #W, U, F, W, launch_variant<RowTile<U, F, W>, W>, \
+    {"u" #U "_f" #F "_w" #W, U, F, W, 17, 0, launch_variant<RowTile<U, F, W>, W>, \
      launch_multi_variant<RowTile<U, F, W>, W>, multi_occupancy<RowTile<U, F, W>, W>}
 #define RPS_VARIANT_Q(U, F, W)                                             \
-    {"q_u" #U "_f" #F "_w" #W, U, F, W,                                   \
+    {"q_u" #U "_f" #F "_w" #W, U, F, W, 17, 0,                            \
      launch_variant<RowTileQ<U, F, W>, W>,                                 \
      launch_multi_variant<RowTileQ<U, F, W>, W>,                           \
      multi_occupancy<RowTileQ<U, F, W>, W>}
 
 #define RPS_VARIANT_W(F, W)                                                \
-    {"w_u2_f" #F "_w" #W, 2, F, W,                                         \
+    {"w_u2_f" #F "_w" #W, 2, F, W, 19, 1,                                  \
      launch_variant<RowTileQ<2, F, W, true>, W>,                           \
      launch_multi_variant<RowTileQ<2, F, W, true>, W>,                     \
      multi_occupancy<RowTileQ<2, F, W, true>, W>}
@@ -1596,11 +1596,8 @@ const VariantInfo kVariants[] = {
     RPS_VARIANT_Q(1, 12, 8),
     RPS_VARIANT_Q(2, 12, 16),
     RPS_VARIANT(2, 14, 8),
-    RPS_VARIANT_W(12, 16),
-    RPS_VARIANT_W(13, 16),
     RPS_VARIANT_W(14, 16),
-    RPS_VARIANT_W(15, 16),
-    RPS_VARIANT_W(16, 12),
+    RPS_VARIANT_W(13, 16),
 };
 
 #undef RPS_VARIANT

@@ -14,6 +14,8 @@ struct VariantInfo {
     int words_per_lane;  // U: a wave covers 64 U words per iteration
     int fast_primes;     // primes ANDed for every word
     int waves;           // waves per workgroup
+    int run_log2;        // row tiles take whole lines up to 2^run_log2 words
+    int wide;            // 16-byte row elements, one workgroup per CU
     hipError_t (*launch)(const SieveParams &params, unsigned grid,
                          hipStream_t stream);
     hipError_t (*launch_multi)(const MultiParams &params, unsigned grid,

@@ -104,6 +104,37 @@ def test_many_overflow_requeues_exactly(fresh_gpu, oracle_lib, pkg, variant):
         assert np.array_equal(num, on) and np.array_equal(den, od)
 
 
+@pytest.mark.parametrize("capacity", [1 << 24, 16384])
+def test_many_mixed_tile_families(fresh_gpu, oracle_lib, pkg, capacity):
+    # With the variant from the tuning table, long-line requests (wide-row
+    # tiles) and short ones run in two launches; survivors still map to the
+    # caller's request indices, equal the plain path's and, with a small
+    # buffer, overflowing requests are searched again exactly.
+    fresh_gpu.set_option("survivor_capacity", capacity)
+    rng = random.Random(51)
+    reqs = random_requests(52, 10, max_bound=5000, max_den=80)
+    for k in (1, 4, 7):
+        coeffs, _ = curves.partially_dense(rng, 12, rng.choice([3, 4, 6]))
+        first = rng.randint(1, 5000)
+        reqs.insert(k, {"coefficients": coeffs, "bound": rng.randint(1_600_000, 3_000_000),
+                        "first": first, "last": first + rng.randint(2, 12)})
+    dense = ["1", str(curves.dense_product())]
+    reqs.insert(5, {"coefficients": dense, "bound": 20000, "first": 1, "last": 2})
+    words = [(2 * r["bound"] + 64) // 64 for r in reqs]
+    assert {pkg.variants()[pkg.tuned_variant(w)][0] for w in words} >= {"q", "w"}
+    announce(case="many-mixed", capacity=capacity, n=len(reqs))
+    results, stats = fresh_gpu.sieve_many(reqs)
+    if capacity == 16384:
+        assert stats["requeued"] >= 1
+    for r, (num, den) in zip(reqs, results):
+        pn, pd = plain(fresh_gpu, r)
+        assert np.array_equal(num, pn) and np.array_equal(den, pd)
+    r = reqs[5]
+    count, digest, on, od = oracle_lib.sieve(r["coefficients"], r["bound"], r["first"],
+                                             r["last"], False, method="words")
+    assert np.array_equal(results[5][0], on) and np.array_equal(results[5][1], od)
+
+
 def test_many_config3_sample(fresh_gpu, workloads):
     # The batched workload's curves (bounds capped to keep the test short):
     # the shared launch equals the per-curve plain loop, and every planted
