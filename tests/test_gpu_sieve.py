@@ -89,6 +89,26 @@ def test_random_curves_match_oracle(fresh_gpu, oracle_lib, seed):
     assert_same(res, oracle_lib.sieve(coeffs, bound, first, last, method="words"))
 
 
+@pytest.mark.parametrize("seed", range(3))
+def test_high_degree_curves_match_oracle(fresh_gpu, oracle_lib, seed):
+    # Degree 50-100 with coefficients of about 1,024 bits (the mode-coverage
+    # shapes): the residue tables come from 101 coefficient residues per
+    # prime; 4-5 live primes keep the survivor sets large.
+    rng = random.Random(300 + seed)
+    degree = rng.choice([50, 63, 75, 100])
+    coeffs, live = curves.partially_dense(rng, rng.randint(4, 5), degree,
+                                          coeff_bound=1 << 700)
+    assert len(coeffs) == degree + 1 and max(len(c) for c in coeffs) > 300
+    bound = rng.choice([1500, 3000, 4096])
+    first = rng.randint(1, 10 ** 6)
+    last = first + 80
+    announce(seed=seed, degree=degree, A=bound, b=f"{first}..{last}")
+    res = fresh_gpu.sieve(coeffs, bound, first, last)
+    want = oracle_lib.sieve(coeffs, bound, first, last, method="words")
+    assert want[0] > 1000
+    assert_same(res, want)
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_square_denominators_match_oracle(fresh_gpu, oracle_lib, seed):
     rng = random.Random(100 + seed)
