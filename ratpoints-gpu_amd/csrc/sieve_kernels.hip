@@ -6,11 +6,14 @@
 //                  p x p 32-bit basis table (basis_kernel, src/sieve.cu:141-172)
 //                  with a 72-byte string per (p, b mod p)
 //   sieve_kernel   the hot loop (src/sieve.cu:221-388 is the kernel it
-//                  replaces): one workgroup per (denominator, word chunk) row
-//                  tile; dense primes ANDed from LDS rows with wave-uniform
-//                  offsets; remaining primes applied to surviving words only;
-//                  survivors compacted with a wave prefix sum and one atomic
-//                  reservation per wave, with a resumable per-wave cursor
+//                  replaces): one workgroup per row tile (a chunk of one line
+//                  or a run of whole lines); dense primes ANDed from LDS rows
+//                  with wave-uniform offsets (8-byte row entries, or 16-byte
+//                  elements read with one ds_read_b128 in the wide tiles);
+//                  remaining primes applied to surviving words only, through
+//                  a per-wave queue; survivors compacted with a wave prefix
+//                  sum and one atomic reservation per drain, with a resumable
+//                  per-wave cursor
 //   direct_kernel  short lines: tiles of whole lines, windows evaluated
 //                  directly from the residue strings
 //   multi_kernel   many curves in one persistent launch: workgroups pull row
@@ -296,51 +299,6 @@ struct RowTile {
         }
     }
 
-    // Prime selection, residue strings and dense rows for denominator b;
-    // every thread of the workgroup calls it.
-    __device__ static void setup(uint64_t *pool, int b, uint64_t ts,
-                                 const LineDesc &L, const int *primes,
-                                 int tid, int lane, int wave) {
-        uint32_t *s_p = reinterpret_cast<uint32_t *>(pool + kSmall);
-        uint32_t *s_c = s_p + kSelected;
-        uint32_t *s_u = s_c + kSelected;
-        uint32_t *s_idx = s_u + kSelected;
-        uint32_t *s_base = s_idx + kSelected;
-        uint32_t *s_magic = s_base + kSelected;
-        uint64_t *cext = pool + kCext;
-        // src/sieve_plan.cpp:75-86: the 32 largest candidates not dividing b.
-        // Lane i tests candidate i; its slot is the number of qualifying
-        // candidates above it.
-        if (wave == 0) {
-            const bool cand = lane < kCandidates;
-            const uint32_t p = cand ? static_cast<uint32_t>(primes[lane]) : 1u;
-            const uint32_t u = cand ? static_cast<uint32_t>(b) % p : 0u;
-            const bool ok = cand && u != 0;
-            const unsigned long long mask = __ballot(ok);
-            if (ok) {
-                const int rank = __popcll(mask >> (lane + 1));
-                if (rank < kSelected) {
-                    s_p[rank] = p;
-                    s_c[rank] = static_cast<uint32_t>(L.neg_bound_mod[lane]);
-                    s_u[rank] = u;
-                    s_idx[rank] = lane;
-                    s_magic[rank] = static_cast<uint32_t>(0x100000000ULL / p);
-                    s_base[rank] = static_cast<uint32_t>(ts % p);
-                }
-            }
-        }
-        __syncthreads();
-        for (int k = tid; k < kSelected * kResidWords; k += kThreads) {
-            const int s = k / kResidWords;
-            const int w = k - s * kResidWords;
-            cext[s * 10 + w] = L.resid[(static_cast<size_t>(s_idx[s]) * kMaxPrime
-                                        + s_u[s]) * kResidWords + w];
-        }
-        __syncthreads();
-        build_rows(pool, tid);
-        __syncthreads();
-    }
-
     // Dense rows from the strings and selection in LDS: entry e of row s is
     // word e of the line mod p, the window of the residue string at
     // (-A + 64 e) mod p; entries e >= p repeat the row so that reads never
@@ -358,13 +316,15 @@ struct RowTile {
             C[s] = uniform(s_c[s]);
             M[s] = uniform(s_magic[s]);
         }
-        // Branch-free: thread t writes entry t of every row, and entry
-        // t + kThreads while that is inside the row's slot (a wave-uniform
-        // test, as kSlot and kThreads are multiples of 64).  Entries from
-        // p + kStride - 1 to the end of the slot are padding the sweep never
-        // reads, so they are written without a test.  All string reads of a
-        // thread are independent and can be in flight together; the window
-        // at bit o is two funnel shifts of three 32-bit string words.
+        // Thread t computes entry t of every row, and entry t + kThreads
+        // while that is below kEntries (a wave-uniform test, as kEntries and
+        // kThreads are multiples of 64).  Narrow rows: entries from p +
+        // kStride - 1 to the end of the slot are padding the sweep never
+        // reads, so they are written without a test.  Wide rows: each entry
+        // goes into the two elements that hold it, when row s has them (a
+        // per-row bound).  All string reads of a thread are independent and
+        // can be in flight together; the window at bit o is two funnel
+        // shifts of three 32-bit string words.
         static_assert(kSlot % 64 == 0 && kEntries % 64 == 0 && kThreads % 64 == 0,
                       "wave-uniform slot tests");
         static_assert(kEntries <= 2 * kThreads, "two entries per thread at most");
@@ -624,12 +584,6 @@ struct RowTileQ {
     __device__ static void wave_range(uint64_t ts, uint64_t te, int wave,
                                       uint64_t &ws, uint64_t &we) {
         Base::wave_range(ts, te, wave, ws, we);
-    }
-
-    __device__ static void setup(uint64_t *pool, int b, uint64_t ts,
-                                 const LineDesc &L, const int *primes,
-                                 int tid, int lane, int wave) {
-        Base::setup(pool, b, ts, L, primes, tid, lane, wave);
     }
 
     // Applies the late primes to the n queued entries (n <= 128) and emits
