@@ -90,15 +90,13 @@ def measured_traffic(variant):
     return data.get("hbm_bytes_per_launch") if data.get("kernel") == variant else None
 
 
-def batched_leg(pkg, wl):
-    """Config 3: 4,096 curves (degrees 3-10, 8-256-bit coefficients, 0-4
-    planted points, A and B log-uniform in [1e3, 1e6]) in one persistent
-    rps_sieve_many launch, against looping rps_sieve curve by curve on the
-    same GPU.  Both must give the same survivor set per curve."""
+def batched_compare(pkg, curves, label):
+    """One rps_sieve_many call over `curves` against looping rps_sieve curve
+    by curve on the same GPU; both must give the same survivor set per
+    curve."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # digest helper only
-    curves = wl.config3_curves()
     sites = sum((2 * c["bound"] + 1) * (c["last"] - c["first"] + 1) for c in curves)
     ctx = pkg.SieveContext(0)
     digests = [(0, 0)] * len(curves)
@@ -112,14 +110,15 @@ def batched_leg(pkg, wl):
         return False
 
     ctx.run_many(curves[:64], lambda q, n, d: False)  # warm tables and code
-    print(f"bench: batched {len(curves)} curves ({sites:.3e} sites)", file=sys.stderr, flush=True)
+    print(f"bench: batched {label}: {len(curves)} curves ({sites:.3e} sites)", file=sys.stderr,
+          flush=True)
     t0 = time.perf_counter()
     rc, stats = ctx.run_many(curves, many_sink)
     many_wall = time.perf_counter() - t0
     many_kernel_ms, many_launches, _ = ctx.kernel_timing()
     loop_counts = [0] * len(curves)
     loop_digests = [(0, 0)] * len(curves)
-    print("bench: plain loop over the same curves", file=sys.stderr, flush=True)
+    print(f"bench: plain loop over the same {label} curves", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     for i, c in enumerate(curves):
         def sink(num, den, i=i):
@@ -131,7 +130,6 @@ def batched_leg(pkg, wl):
     ctx.close()
     same = counts == loop_counts and digests == loop_digests
     return {
-        "workload": "config 3: 4096 planted curves, seed 3",
         "curves": len(curves),
         "sites": sites,
         "batched_curves_per_s": round(len(curves) / many_wall, 1),
@@ -146,6 +144,21 @@ def batched_leg(pkg, wl):
         "survivors": sum(counts),
         "identical_to_plain_loop": same,
     }
+
+
+def batched_leg(pkg, wl):
+    """Config 3: 4,096 curves (degrees 3-10, 8-256-bit coefficients, 0-4
+    planted points, A and B log-uniform in [1e3, 1e6]) in one rps_sieve_many
+    call against looping rps_sieve curve by curve.  Also the same curves cut
+    to A, B <= 1,000 (config-1-sized searches, about 2e6 sites each), where
+    a call's fixed cost dominates the loop."""
+    curves = wl.config3_curves()
+    result = {"workload": "config 3: 4096 planted curves, seed 3"}
+    result.update(batched_compare(pkg, curves, "config 3"))
+    small = [dict(c, bound=min(c["bound"], 1000), last=min(c["last"], 1000)) for c in curves]
+    result["small"] = {"workload": "config 3 curves with A, B <= 1000"}
+    result["small"].update(batched_compare(pkg, small, "small"))
+    return result
 
 
 def dense_primes(variant):
@@ -366,7 +379,8 @@ def main():
         result["error"] = f"planted points missing: {missing}"
     if not args.no_batched:
         result["batched"] = batched_leg(pkg, wl)
-        if not result["batched"]["identical_to_plain_loop"]:
+        if not (result["batched"]["identical_to_plain_loop"]
+                and result["batched"]["small"]["identical_to_plain_loop"]):
             result["error"] = "batched survivors differ from the plain loop"
     if args.modes:
         result["modes"] = modes_leg(wl)
