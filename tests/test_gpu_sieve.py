@@ -75,6 +75,22 @@ def test_reference_survivor_counts(fresh_gpu, name):
     assert stats["survivor_count"] == row["survivors"]
 
 
+# The reference's remaining published counts, at their full sizes (up to
+# 2.6e14 sites, a few seconds each): the record curve over 1e7 x 1e7 and the
+# rank-31 cubics at A = 1e8 and A = 2e9 (lines of up to 62.5 million words,
+# cut into chunks).
+FULL_GOLDEN = sorted(set(ROWS) - set(FAST_GOLDEN))
+
+
+@pytest.mark.parametrize("name", FULL_GOLDEN)
+def test_reference_survivor_counts_full_size(fresh_gpu, name):
+    row = ROWS[name]
+    announce(case=name, A=row["bound"], b=f"{row['first']}..{row['last']}")
+    rc, stats = fresh_gpu.run(row["coefficients"].split(), row["bound"], row["first"],
+                              row["last"], row["square"], lambda n, d: False)
+    assert stats["survivor_count"] == row["survivors"]
+
+
 @pytest.mark.parametrize("seed", range(10))
 def test_random_curves_match_oracle(fresh_gpu, oracle_lib, seed):
     rng = random.Random(seed)
