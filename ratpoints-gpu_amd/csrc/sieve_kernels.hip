@@ -1539,7 +1539,7 @@ hipError_t multi_occupancy(int *blocks) {
      multi_occupancy<RowTileQ<U, F, W>, W>}
 
 #define RPS_VARIANT_W(F, W)                                                \
-    {"w_u2_f" #F "_w" #W, 2, F, W, 19, 1,                                  \
+    {"w_u2_f" #F "_w" #W, 2, F, W, 20, 1,                                  \
      launch_variant<RowTileQ<2, F, W, true>, W>,                           \
      launch_multi_variant<RowTileQ<2, F, W, true>, W>,                     \
      multi_occupancy<RowTileQ<2, F, W, true>, W>}
