@@ -147,16 +147,16 @@ def batched_compare(pkg, curves, label):
 
 
 def batched_leg(pkg, wl):
-    """Config 3: 4,096 curves (degrees 3-10, 8-256-bit coefficients, 0-4
+    """Batched: 4,096 curves (degrees 3-10, 8-256-bit coefficients, 0-4
     planted points, A and B log-uniform in [1e3, 1e6]) in one rps_sieve_many
     call against looping rps_sieve curve by curve.  Also the same curves cut
-    to A, B <= 1,000 (config-1-sized searches, about 2e6 sites each), where
+    to A, B <= 1,000 (searches of about 2e6 sites each), where
     a call's fixed cost dominates the loop."""
-    curves = wl.config3_curves()
-    result = {"workload": "config 3: 4096 planted curves, seed 3"}
-    result.update(batched_compare(pkg, curves, "config 3"))
+    curves = wl.batched_curves()
+    result = {"workload": "4096 planted curves, seed 3 (degrees 3-10, A and B log-uniform in [1e3, 1e6])"}
+    result.update(batched_compare(pkg, curves, "4096-curve"))
     small = [dict(c, bound=min(c["bound"], 1000), last=min(c["last"], 1000)) for c in curves]
-    result["small"] = {"workload": "config 3 curves with A, B <= 1000"}
+    result["small"] = {"workload": "the same 4096 curves with A, B <= 1000"}
     result["small"].update(batched_compare(pkg, small, "small"))
     return result
 
@@ -170,11 +170,11 @@ def dense_primes(variant):
 
 
 def serving_leg(wl, count, qps):
-    """Config 5: Poisson arrivals of `count` requests at `qps` over the
+    """Serving: Poisson arrivals of `count` requests at `qps` over the
     seeded serving mix, through bin/ratpoints_serve (continuous batching)
     and through the one-request-at-a-time loop on the same GPU; per-request
     point lists must be identical."""
-    reqs = wl.config5_requests(seed=5, count=count, qps=qps)
+    reqs = wl.serving_requests(seed=5, count=count, qps=qps)
     sites = sum((2 * r["bound"] + 1) * (r["last"] - r["first"] + 1) for r in reqs)
     summaries, same = run_serve_modes(wl, reqs, "serving")
 
@@ -183,7 +183,7 @@ def serving_leg(wl, count, qps):
                                   "sites", "gsites_per_s", "rounds", "errors", "points")}
 
     return {
-        "workload": "config 5: %d requests, Poisson %.1f/s, seed 5 (degrees 2-12, "
+        "workload": "%d requests, Poisson %.1f/s, seed 5 (degrees 2-12, "
                     "16-512-bit coefficients, height log-uniform 1e2..1e7, 30%% -1, "
                     "10%% intervals)" % (count, qps),
         "box_sites": sites,
@@ -221,17 +221,17 @@ def run_serve_modes(wl, reqs, label, extra=()):
 
 
 def modes_leg(wl):
-    """Config 4: 1,024 curves at once -- 512 cubics/quartics (64-128-bit
+    """Mode coverage: 1,024 curves at once -- 512 cubics/quartics (64-128-bit
     coefficients) with square denominators k <= 46,340 and A = 1e7, 512 of
     degree 50-100 with 1,024-bit coefficients at height 1e5, each with 1-3
     x-intervals -- through the serving loop (all admitted together) and one
     search at a time."""
-    reqs = wl.config4_requests()
+    reqs = wl.mode_coverage_requests()
     sites = sum((2 * r["bound"] + 1) * (r["last"] - r["first"] + 1) for r in reqs)
     summaries, same = run_serve_modes(wl, reqs, "modes")
     serve, alone = summaries["serve"], summaries["alone"]
     return {
-        "workload": "config 4: 1024 curves, seed 4 (512 square-denominator cubics/quartics "
+        "workload": "1024 curves, seed 4 (512 square-denominator cubics/quartics "
                     "k<=46340 A=1e7; 512 degree 50-100, 1024-bit, height 1e5; 1-3 intervals each)",
         "box_sites": sites,
         "batched_curves_per_s": round(len(reqs) / serve["makespan_s"], 1),
@@ -284,7 +284,7 @@ def main():
         sys.exit(2)
     pkg = load("ratpoints_gpu_amd", os.path.join(ROOT, "ratpoints-gpu_amd", "__init__.py"))
     wl = load("ratpoints_gpu_amd_workloads", os.path.join(ROOT, "ratpoints-gpu_amd", "workloads.py"))
-    coeffs, planted = wl.config2_curve()
+    coeffs, planted = wl.sextic_a5e6_curve()
     scoeffs = [str(c) for c in coeffs]
     total_batches = (LAST + BATCH - 1) // BATCH
     steps = max(1, min(args.steps, total_batches))

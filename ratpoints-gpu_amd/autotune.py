@@ -36,7 +36,7 @@ def load(name, path):
 def measure(sites_per_point=3e11):
     pkg = load("ratpoints_gpu_amd", os.path.join(HERE, "__init__.py"))
     wl = load("ratpoints_gpu_amd_workloads", os.path.join(HERE, "workloads.py"))
-    coeffs = [str(c) for c in wl.config2_curve()[0]]
+    coeffs = [str(c) for c in wl.sextic_a5e6_curve()[0]]
     names = pkg.variants()
     rows = []
     with pkg.SieveContext(0) as ctx:

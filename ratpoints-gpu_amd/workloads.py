@@ -144,21 +144,21 @@ def planted_curve(seed, q_degree, q_bound, point_bound, n_points, r_bound=100,
             return f, points
 
 
-def config1_curve(seed=1):
+def sextic_a1000_curve(seed=1):
     """Plain method, small: sextic, q coefficients in [-2^15, 2^15], six
     planted points with |a_i| <= 1000, 1 <= b_i <= 1000; box A = B = 1000."""
     return planted_curve(seed, 3, 2 ** 15, 1000, 6)
 
 
-def config2_curve(seed=2):
+def sextic_a5e6_curve(seed=2):
     """Plain method at full size: q coefficients in [-2^30, 2^30], planted
     |a_i|, b_i <= 5e6 (coefficients of roughly 150 bits); box A = B = 5e6."""
     return planted_curve(seed, 3, 2 ** 30, 5_000_000, 6)
 
 
-CONFIG1 = {"name": "planted-sextic-A1000-B1000", "seed": 1, "bound": 1000,
+SEXTIC_A1000 = {"name": "planted-sextic-A1000-B1000", "seed": 1, "bound": 1000,
            "first": 1, "last": 1000}
-CONFIG2 = {"name": "planted-sextic-A5e6-B5e6", "seed": 2, "bound": 5_000_000,
+SEXTIC_A5E6 = {"name": "planted-sextic-A5e6-B5e6", "seed": 2, "bound": 5_000_000,
            "first": 1, "last": 5_000_000, "batch": 65536}
 
 RECORD_CURVE = [247747600, -985905640, 567207969, 2396040466, 52485681,
@@ -214,7 +214,7 @@ def batched_curve(rng, degree, bits, n_points, bound, den_bound):
             return f, points
 
 
-def config3_curves(seed=3, count=4096):
+def batched_curves(seed=3, count=4096):
     """Batched form: degrees uniform over {3, 4, 5, 6, 8, 10}, coefficient
     size uniform in 8..256 bits, 0-4 planted points, numerator and
     denominator bounds independently log-uniform in [1e3, 1e6].
@@ -237,7 +237,7 @@ def _unit(rng):
     return (rng.next() >> 11) / float(1 << 53)
 
 
-def config5_requests(seed=5, count=64, qps=2.0, min_height=100, max_height=10_000_000):
+def serving_requests(seed=5, count=64, qps=2.0, min_height=100, max_height=10_000_000):
     """Serving mix: Poisson arrivals at `qps` requests/s; degree uniform in
     2..12, coefficient size uniform in 16..512 bits, height H log-uniform in
     [min_height, max_height] (|a| <= H, 1 <= b <= H), 0-2 planted points
@@ -294,7 +294,7 @@ def _random_intervals(rng, count):
     return out
 
 
-def config4_requests(seed=4, count=1024, square_fraction=0.5, square_k=46340,
+def mode_coverage_requests(seed=4, count=1024, square_fraction=0.5, square_k=46340,
                      square_bound=10_000_000, high_height=100_000):
     """Mode-coverage batch: `count` curves, all arriving at once.
     * square part: cubic and quartic curves with 64-128-bit coefficients,
@@ -304,7 +304,7 @@ def config4_requests(seed=4, count=1024, square_fraction=0.5, square_k=46340,
     * high-degree part: degree 50-100, random 1,024-bit coefficients, height
       high_height (|a| <= H, b <= H), squarefree by construction check.
     Every curve carries 1-3 closed x-intervals; points at infinity enabled.
-    Returns request dicts (see config5_requests) with key "square"."""
+    Returns request dicts (see serving_requests) with key "square"."""
     rng = SplitMix64(seed)
     n_square = int(round(count * square_fraction))
     out = []
@@ -364,18 +364,18 @@ def config4_requests(seed=4, count=1024, square_fraction=0.5, square_k=46340,
 
 
 def main(argv=None):
-    """Writes a serving trace: python workloads.py config5 OUT [COUNT QPS]
-    or python workloads.py config4 OUT."""
+    """Writes a serving trace: python workloads.py serving OUT [COUNT QPS]
+    or python workloads.py modes OUT."""
     import sys
     args = list(sys.argv[1:] if argv is None else argv)
-    if len(args) < 2 or args[0] not in ("config4", "config5"):
-        raise SystemExit("usage: workloads.py config5 OUT [COUNT QPS] | config4 OUT")
-    if args[0] == "config5":
+    if len(args) < 2 or args[0] not in ("modes", "serving"):
+        raise SystemExit("usage: workloads.py serving OUT [COUNT QPS] | modes OUT")
+    if args[0] == "serving":
         count = int(args[2]) if len(args) > 2 else 100
         qps = float(args[3]) if len(args) > 3 else 5.0
-        reqs = config5_requests(count=count, qps=qps)
+        reqs = serving_requests(count=count, qps=qps)
     else:
-        reqs = config4_requests()
+        reqs = mode_coverage_requests()
     write_trace(args[1], reqs)
     print("wrote %d requests to %s" % (len(reqs), args[1]))
 

@@ -267,14 +267,14 @@ def test_record_curve_published_points():
 
 
 @pytest.mark.gpu
-def test_config1_planted_points(workloads):
-    coeffs, points = workloads.config1_curve()
+def test_sextic_a1000_planted_points(workloads):
+    coeffs, points = workloads.sextic_a1000_curve()
     text = " ".join(map(str, coeffs))
     proc = run(PRODUCT, [text, "1000", "-q", "-i", "-f", "%x %y %z\\n"])
     found = {(int(a), int(b)) for a, _, b in
              (line.split() for line in proc.stdout.decode().splitlines())}
     assert all(p in found for p in points)
-    # Byte-exact against the brute-force oracle on the full config-1 box
+    # Byte-exact against the brute-force oracle on the full A = B = 1,000 box
     # would take minutes in Python; the verified set is compared with the
     # oracle's exact stage on the first 60 denominators instead.
     sub = run(PRODUCT, [text, "1000", "-du", "60", "-q"])

@@ -135,16 +135,16 @@ def test_many_mixed_tile_families(fresh_gpu, oracle_lib, pkg, capacity):
     assert np.array_equal(results[5][0], on) and np.array_equal(results[5][1], od)
 
 
-def test_many_config3_sample(fresh_gpu, workloads):
+def test_many_batched_sample(fresh_gpu, workloads):
     # The batched workload's curves (bounds capped to keep the test short):
     # the shared launch equals the per-curve plain loop, and every planted
     # point survives.
-    sample = workloads.config3_curves(count=96)
+    sample = workloads.batched_curves(count=96)
     for c in sample:
         c["bound"] = min(c["bound"], 30000)
         c["last"] = min(c["last"], 2000)
         c["points"] = [(a, b) for a, b in c["points"] if abs(a) <= c["bound"] and b <= c["last"]]
-    announce(case="config3-sample", n=len(sample))
+    announce(case="batched-sample", n=len(sample))
     results, stats = fresh_gpu.sieve_many(sample)
     for c, (num, den) in zip(sample, results):
         pn, pd = plain(fresh_gpu, c)

@@ -315,7 +315,7 @@ def test_monotone_in_bound(fresh_gpu):
 
 
 def test_planted_points_survive(fresh_gpu, workloads):
-    coeffs, points = workloads.config2_curve()
+    coeffs, points = workloads.sextic_a5e6_curve()
     announce(case="planted", A=5_000_000)
     res = fresh_gpu.sieve(coeffs, 5_000_000, 1, 2000)
     got = set(zip(res.numerators.tolist(), res.denominators.tolist()))
@@ -370,9 +370,9 @@ def test_short_lines_both_modes_match_oracle(fresh_gpu, oracle_lib, bound):
 
 
 def test_short_lines_many_denominators(fresh_gpu, oracle_lib, workloads):
-    # Config 1 in full: the planted sextic on [-1000, 1000] x [1, 1000].
-    coeffs, points = workloads.config1_curve()
-    announce(case="config1")
+    # The planted sextic on [-1000, 1000] x [1, 1000] in full.
+    coeffs, points = workloads.sextic_a1000_curve()
+    announce(case="sextic-A1000")
     res = fresh_gpu.sieve(coeffs, 1000, 1, 1000)
     assert_same(res, oracle_lib.sieve(coeffs, 1000, 1, 1000, method="words"))
     got = set(zip(res.numerators.tolist(), res.denominators.tolist()))

@@ -231,7 +231,7 @@ def test_render_output_options():
 
 
 def test_planted_curves(workloads):
-    for make in (workloads.config1_curve, workloads.config2_curve):
+    for make in (workloads.sextic_a1000_curve, workloads.sextic_a5e6_curve):
         coeffs, points = make()
         assert len(coeffs) == 7 and workloads.is_squarefree(coeffs)
         for a, b in points:
@@ -240,9 +240,9 @@ def test_planted_curves(workloads):
 
 
 @pytest.mark.slow
-def test_config1_planted_points_survive(oracle_lib, workloads):
-    coeffs, points = workloads.config1_curve()
-    c = workloads.CONFIG1
+def test_sextic_a1000_planted_points_survive(oracle_lib, workloads):
+    coeffs, points = workloads.sextic_a1000_curve()
+    c = workloads.SEXTIC_A1000
     _, _, num, den = oracle_lib.sieve(coeffs, c["bound"], c["first"], c["last"], method="words")
     found = set(zip(num.tolist(), den.tolist()))
     assert all(p in found for p in points)

@@ -63,7 +63,7 @@ def expected_points(r):
 
 
 def small_trace(workloads, seed, count, max_height, qps):
-    reqs = workloads.config5_requests(seed=seed, count=count, qps=qps, min_height=20,
+    reqs = workloads.serving_requests(seed=seed, count=count, qps=qps, min_height=20,
                                       max_height=max_height)
     t = reqs[-1]["arrival_ms"]
     extra = [
@@ -156,14 +156,14 @@ def test_serve_matches_alone_on_gpu(pkg, workloads, tmp_path):
 
 
 def mode_trace(workloads, count, **sizes):
-    reqs = workloads.config4_requests(seed=44, count=count, **sizes)
+    reqs = workloads.mode_coverage_requests(seed=44, count=count, **sizes)
     for r in reqs:
         r["arrival_ms"] = 0.0
     return reqs
 
 
 def test_mode_coverage_batch_on_double(workloads, tmp_path):
-    # Config-4 shapes at oracle-sized boxes: square-denominator cubics and
+    # Mode-coverage shapes at oracle-sized boxes: square-denominator cubics and
     # quartics (b = k^2) and degree 50-100 curves with 1,024-bit
     # coefficients, each with 1-3 x-intervals, all submitted at once.
     if not os.path.exists(DOUBLE):
