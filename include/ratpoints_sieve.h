@@ -147,7 +147,8 @@ typedef int (*rps_multi_sink)(void *user, const int32_t *request,
  * counter, so short searches share the GPU instead of each paying a launch
  * and a synchronisation.  With the variant left to the tuning table,
  * requests whose lines take wide-row tiles (one workgroup per CU) get a
- * second launch of their own.  Each request's survivor set equals what rps_sieve
+ * launch of their own; more than 8,192 requests are cut into launches of at
+ * most 8,192 (each keeps one 1.33 MB residue table per curve on the device).  Each request's survivor set equals what rps_sieve
  * gives for it alone.  A request whose survivors overflow the buffer during
  * the shared launch is searched again alone (stats->requeued counts them).
  * No counterpart in the reference, which searches one curve per process

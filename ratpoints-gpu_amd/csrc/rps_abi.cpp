@@ -1308,26 +1308,34 @@ void add_stats(rps_stats &into, const rps_stats &s) {
 
 extern "C" {
 
+// Requests per launch at most: the launch keeps one residue table per
+// distinct curve on the device (1.33 MB each, 10.9 GB for this many).
+constexpr size_t kMaxCurvesPerLaunch = 8192;
+
 // With the variant left to the tuning table, requests whose lines want a
 // one-workgroup-per-CU (wide-row) tile run in a launch of their own, and
 // the others share a launch of the tuning table's choice for the largest of
 // them: a launch has one LDS layout, and the wide one would cost the short
-// lines half their occupancy.
+// lines half their occupancy.  Each group is cut into launches of at most
+// kMaxCurvesPerLaunch requests.
 int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
                    rps_multi_sink sink, void *user, rps_stats *stats) {
     if (!ctx) {
         return RPS_EINVAL;
     }
-    if (ctx->variant >= 0 || n_requests < 2 || !requests || !sink
-        || n_requests > (size_t(1) << 20)) {
+    if (n_requests < 2 || !requests || !sink || n_requests > (size_t(1) << 20)) {
         return sieve_many_launch(ctx, requests, n_requests, sink, user, stats, -1);
     }
-    std::vector<int32_t> groups[2];  // [0] two-workgroup tiles, [1] wide
+    std::vector<int32_t> family[2];  // [0] two-workgroup tiles (or forced), [1] wide
     try {
         for (size_t r = 0; r < n_requests; ++r) {
             validate(&requests[r]);
-            const int v = tuned_variant(line_geometry(&requests[r]).words);
-            groups[rps::variant(v)->wide ? 1 : 0].push_back(static_cast<int32_t>(r));
+            int f = 0;
+            if (ctx->variant < 0) {
+                const int v = tuned_variant(line_geometry(&requests[r]).words);
+                f = rps::variant(v)->wide ? 1 : 0;
+            }
+            family[f].push_back(static_cast<int32_t>(r));
         }
     } catch (const ApiError &e) {
         ctx->last_kernel_ms = 0.0;
@@ -1338,8 +1346,16 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
         }
         return fail(ctx, e);
     }
-    if (groups[0].empty() || groups[1].empty()) {
+    if (n_requests <= kMaxCurvesPerLaunch && (family[0].empty() || family[1].empty())) {
         return sieve_many_launch(ctx, requests, n_requests, sink, user, stats, -1);
+    }
+    std::vector<std::vector<int32_t>> groups;
+    for (const std::vector<int32_t> &f : family) {
+        for (size_t at = 0; at < f.size(); at += kMaxCurvesPerLaunch) {
+            const size_t end = std::min(f.size(), at + kMaxCurvesPerLaunch);
+            groups.emplace_back(f.begin() + static_cast<std::ptrdiff_t>(at),
+                                f.begin() + static_cast<std::ptrdiff_t>(end));
+        }
     }
     rps_stats total{};
     double kernel_ms = 0.0, sites = 0.0;

@@ -135,6 +135,32 @@ def test_many_mixed_tile_families(fresh_gpu, oracle_lib, pkg, capacity):
     assert np.array_equal(results[5][0], on) and np.array_equal(results[5][1], od)
 
 
+def test_many_more_requests_than_one_launch_holds(fresh_gpu, oracle_lib):
+    # More than 8,192 requests are cut into several launches (one residue
+    # table per curve per launch); survivors still carry the caller's
+    # request indices.  Dense requests at both ends make the mapping visible.
+    rng = random.Random(61)
+    dense = ["1", str(curves.dense_product())]
+    reqs = []
+    for i in range(8300):
+        if i % 1000 == 7:
+            reqs.append({"coefficients": dense, "bound": rng.randint(1, 40), "first": 1,
+                         "last": rng.randint(1, 3)})
+        else:
+            coeffs, _ = curves.partially_dense(rng, 3, rng.choice([2, 3, 4]), coeff_bound=50)
+            first = rng.randint(1, 500)
+            reqs.append({"coefficients": coeffs, "bound": rng.randint(1, 40), "first": first,
+                         "last": first + rng.randint(0, 3)})
+    announce(case="many-8300", n=len(reqs))
+    results, stats = fresh_gpu.sieve_many(reqs)
+    assert stats["survivor_count"] == sum(len(n) for n, _ in results)
+    for i in list(range(0, 8300, 97)) + [7, 8007, 8299]:
+        r = reqs[i]
+        count, digest, on, od = oracle_lib.sieve(r["coefficients"], r["bound"], r["first"],
+                                                 r["last"], False, method="words")
+        assert np.array_equal(results[i][0], on) and np.array_equal(results[i][1], od), i
+
+
 def test_many_batched_sample(fresh_gpu, workloads):
     # The batched workload's curves (bounds capped to keep the test short):
     # the shared launch equals the per-curve plain loop, and every planted
