@@ -401,6 +401,15 @@ struct RowTile {
                 }
             }
         }
+        // The scalar offset updates run while the reads are in flight: the
+        // empty asm takes the new offsets as SGPR operands here, so they are
+        // not sunk behind the AND and the queue branch into the loop latch.
+#pragma unroll
+        for (int s = 0; s < FAST; ++s) {
+            const uint32_t next = off[s] + kStride;
+            off[s] = next >= prime[s] ? next - prime[s] : next;
+            asm volatile("" ::"s"(off[s]));
+        }
         uint32_t lo[U], hi[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -423,11 +432,6 @@ struct RowTile {
                     hi[u] &= static_cast<uint32_t>(x[s][u] >> 32);
                 }
             }
-        }
-#pragma unroll
-        for (int s = 0; s < FAST; ++s) {
-            const uint32_t next = off[s] + kStride;
-            off[s] = next >= prime[s] ? next - prime[s] : next;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
