@@ -52,7 +52,9 @@ typedef struct rps_ctx rps_ctx;
 typedef struct rps_request {
     const char *const *coefficients; /* decimal c0 .. cn, optional sign     */
     int32_t n_coefficients;          /* n + 1, 2..101                       */
-    int64_t numerator_bound;         /* A >= 1: numerators a in [-A, A]     */
+    int64_t numerator_bound;         /* A >= 1: numerators a in [-A, A];
+                                        A < 2^52 (lines of at most 2^47
+                                        64-bit words), else RPS_EINVAL      */
     int32_t den_first;               /* first denominator parameter, >= 1   */
     int32_t den_last;                /* last parameter, inclusive           */
     int32_t square_denominators;     /* nonzero: b = t^2, den_last <= 46340 */

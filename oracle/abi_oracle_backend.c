@@ -232,8 +232,12 @@ int rps_check_request(const rps_request *req, char *message, size_t message_len)
     if (!req || req->n_coefficients < 2 || req->n_coefficients > 101 || !req->coefficients) {
         reason = "supported polynomial degrees are 1 through 100";
         rc = RPS_EINVAL;
-    } else if (req->numerator_bound < 1) {
+    } else if (req->numerator_bound < 1 || req->numerator_bound > (INT64_MAX - 32) / 2) {
         reason = "invalid numerator bound";
+        rc = RPS_EINVAL;
+    } else if ((2 * (uint64_t)req->numerator_bound + 64) / 64 > ((uint64_t)1 << 47)) {
+        /* the device library's line limit: 2^47 64-bit words */
+        reason = "maximum height exceeds GPU limit";
         rc = RPS_EINVAL;
     } else if (req->den_first < 1 || req->den_first > req->den_last
                || (req->square_denominators && req->den_last > 46340)) {

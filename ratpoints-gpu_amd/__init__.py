@@ -109,12 +109,25 @@ def load_library(path=LIB_PATH):
     lib.rps_tuned_variant.argtypes = [ctypes.c_uint64]
     lib.rps_device_count.restype = ctypes.c_int
     lib.rps_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    lib.rps_check_request.restype = ctypes.c_int
+    lib.rps_check_request.argtypes = [ctypes.POINTER(RpsRequest), ctypes.c_char_p,
+                                      ctypes.c_size_t]
     lib.rps_last_kernel_timing.restype = ctypes.c_int
     lib.rps_last_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_uint64),
                                            ctypes.POINTER(ctypes.c_double)]
     _LIB = lib
     return lib
+
+
+def check_request(coefficients, bound, first, last, square=False):
+    """rps_check_request: (status code, reason) without touching a device."""
+    lib = load_library()
+    coef = (ctypes.c_char_p * len(coefficients))(*[str(c).encode() for c in coefficients])
+    req = RpsRequest(coef, len(coefficients), bound, first, last, 1 if square else 0)
+    message = ctypes.create_string_buffer(256)
+    rc = lib.rps_check_request(ctypes.byref(req), message, len(message))
+    return rc, message.value.decode()
 
 
 def variants():

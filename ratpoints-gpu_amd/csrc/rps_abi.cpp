@@ -15,6 +15,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <numeric>
 #include <string>
@@ -230,6 +231,9 @@ void ensure_capacity(rps_ctx *ctx) {
     if (ctx->allocated_capacity == ctx->capacity) {
         return;
     }
+    // Until both slots hold buffers of the new size, no size is recorded: a
+    // failed allocation must not leave a slot without buffers looking ready.
+    ctx->allocated_capacity = 0;
     for (Slot &s : ctx->slots) {
         if (s.d_num) {
             RPS_HIP(hipFree(s.d_num));
@@ -273,6 +277,12 @@ void validate(const rps_request *req) {
     }
     if (req->numerator_bound < 1 || req->numerator_bound > (LLONG_MAX - 32) / 2) {
         throw ApiError{RPS_EINVAL, "invalid numerator bound"};
+    }
+    // A line is at most 2^22 tiles of at most 2^25 words (2^47 words,
+    // A < 4.5e15): the same limit on every path, so a request that passes
+    // here never fails later for its size.
+    if ((2ULL * static_cast<unsigned long long>(req->numerator_bound) + 64) / 64 > (1ULL << 47)) {
+        throw ApiError{RPS_EINVAL, "maximum height exceeds GPU limit"};
     }
     // src/sieve_plan.cpp:55-58
     if (req->den_first < 1 || req->den_first > req->den_last
@@ -859,6 +869,12 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
     } catch (const std::bad_alloc &) {
         ctx->error = "out of host memory";
         return RPS_ENOMEM;
+    } catch (const std::exception &e) {
+        // Nothing may unwind through the C ABI.
+        ctx->error = std::string("internal error: ") + e.what();
+        (void)hipStreamSynchronize(ctx->compute);
+        (void)hipStreamSynchronize(ctx->copy);
+        return RPS_EINTERNAL;
     }
 }
 
@@ -932,7 +948,9 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
 
         // Geometry and tiling per curve.  Long lines are cut into chunks of
         // at most 2^18 words so one curve's tiles stay small next to the rest
-        // of the launch.
+        // of the launch; lines of more than 2^40 words take longer chunks
+        // (at most 2^22 per line, of at most 2^25 words, as on the plain
+        // path).
         std::vector<rps::LineDesc> lines(n);
         std::vector<double> cost(n);
         double sites = 0.0;
@@ -943,8 +961,11 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
             if (L.words <= ctx->direct_words) {
                 direct_tiling(L, L.n_t);
             } else {
-                const unsigned long long chunks = std::max(
-                    (L.words + (1ULL << 18) - 1) >> 18, 1ULL);
+                const unsigned long long chunks = std::min(
+                    std::max((L.words + (1ULL << 18) - 1) >> 18, 1ULL), kMaxTilesPerLaunch);
+                if ((L.words + chunks - 1) / chunks > (1ULL << 25)) {
+                    throw ApiError{RPS_EINVAL, "maximum height exceeds GPU limit"};
+                }
                 row_tiling(L, chunks, stride);
                 row_runs(L, L.n_t, var->run_log2);
             }
@@ -972,10 +993,15 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
         const int cap = static_cast<int>(std::max<long long>(
             ctx->table_slots, static_cast<long long>(distinct.size())));
         if (cap > ctx->t_cap) {
+            // Empty the cache before the old tables go: if the larger
+            // allocation fails, no slot may point into freed memory.
+            ctx->t_cap = 0;
+            ctx->t_index.clear();
+            ctx->t_key.clear();
+            ctx->t_used.clear();
             ctx->t_resid.release();
             ctx->t_resid.ensure(static_cast<size_t>(cap) * rps::kResidTableWords);
             ctx->t_cap = cap;
-            ctx->t_index.clear();
             ctx->t_key.assign(cap, {});
             ctx->t_used.assign(cap, 0);
         }
@@ -1059,7 +1085,10 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
             C.line.neg_bound_mod = ctx->m_negmod.ptr + static_cast<size_t>(k) * rps::kCandidates;
             C.request = r;
             C.tile_start = tile_start[k];
-            tile_start[k + 1] = tile_start[k] + tile_count(C.line);
+            if (__builtin_add_overflow(tile_start[k], tile_count(C.line), &tile_start[k + 1])
+                || tile_start[k + 1] > (1ULL << 62)) {
+                throw ApiError{RPS_EINVAL, "too much work in one call"};
+            }
         }
         const unsigned long long total_tiles = tile_start[n];
         // Curve of every 2^shift-th tile (at most 2^20 buckets), so the
@@ -1267,6 +1296,12 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
     } catch (const std::bad_alloc &) {
         ctx->error = "out of host memory";
         return RPS_ENOMEM;
+    } catch (const std::exception &e) {
+        // Nothing may unwind through the C ABI.
+        ctx->error = std::string("internal error: ") + e.what();
+        (void)hipStreamSynchronize(ctx->compute);
+        (void)hipStreamSynchronize(ctx->copy);
+        return RPS_EINTERNAL;
     }
 }
 
@@ -1304,10 +1339,6 @@ void add_stats(rps_stats &into, const rps_stats &s) {
     into.requeued += s.requeued;
 }
 
-}  // namespace
-
-extern "C" {
-
 // Requests per launch at most: the launch keeps one residue table per
 // distinct curve on the device (1.33 MB each, 10.9 GB for this many).
 constexpr size_t kMaxCurvesPerLaunch = 8192;
@@ -1318,11 +1349,8 @@ constexpr size_t kMaxCurvesPerLaunch = 8192;
 // them: a launch has one LDS layout, and the wide one would cost the short
 // lines half their occupancy.  Each group is cut into launches of at most
 // kMaxCurvesPerLaunch requests.
-int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
-                   rps_multi_sink sink, void *user, rps_stats *stats) {
-    if (!ctx) {
-        return RPS_EINVAL;
-    }
+int sieve_many_split(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
+                     rps_multi_sink sink, void *user, rps_stats *stats) {
     if (n_requests < 2 || !requests || !sink || n_requests > (size_t(1) << 20)) {
         return sieve_many_launch(ctx, requests, n_requests, sink, user, stats, -1);
     }
@@ -1384,6 +1412,26 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
         *stats = total;
     }
     return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rps_sieve_many(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
+                   rps_multi_sink sink, void *user, rps_stats *stats) {
+    if (!ctx) {
+        return RPS_EINVAL;
+    }
+    try {
+        return sieve_many_split(ctx, requests, n_requests, sink, user, stats);
+    } catch (const std::bad_alloc &) {
+        ctx->error = "out of host memory";
+        return RPS_ENOMEM;
+    } catch (const std::exception &e) {
+        ctx->error = std::string("internal error: ") + e.what();
+        return RPS_EINTERNAL;
+    }
 }
 
 }  // extern "C"

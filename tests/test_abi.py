@@ -72,3 +72,20 @@ def test_missing_library_fails_loudly(pkg, tmp_path):
             pkg.load_library(str(tmp_path / "nope.so"))
     finally:
         pkg._LIB = saved
+
+
+def test_check_request_limits(pkg):
+    # Host-side validation only: the same limits as every sieve entry point.
+    ok = (["1", "0", "1"], 1000, 1, 10)
+    assert pkg.check_request(*ok) == (0, "")
+    assert pkg.check_request(["1", "0", "1"], 2 ** 52 - 1, 1, 10) == (0, "")
+    assert pkg.check_request(["1", "0", "1"], 2 ** 52, 1, 10) == (
+        1, "maximum height exceeds GPU limit")
+    assert pkg.check_request(["1", "0", "1"], 0, 1, 10) == (1, "invalid numerator bound")
+    assert pkg.check_request(["1", "0", "1"], 10, 1, 46341, True) == (
+        1, "invalid denominator range")
+    assert pkg.check_request(["1", "0", "1"], 10, 1, 46340, True) == (0, "")
+    assert pkg.check_request(["1"], 10, 1, 5)[0] == 1
+    assert pkg.check_request(["1"] * 102, 10, 1, 5)[0] == 1
+    assert pkg.check_request(["1", "x"], 10, 1, 5) == (
+        1, "coefficients must be decimal integers")

@@ -343,6 +343,16 @@ def test_invalid_requests(fresh_gpu, pkg):
     with pytest.raises(pkg.SieveError) as err:
         fresh_gpu.run(["1", "1"], 10, 1, 46341, True, lambda n, d: False)
     assert str(err.value) == "invalid denominator range"
+    # Heights whose lines exceed 2^47 words are refused up front, on the
+    # plain and the many-curve path alike.
+    with pytest.raises(pkg.SieveError) as err:
+        fresh_gpu.run(["1", "1"], 2 ** 52, 1, 1, False, lambda n, d: False)
+    assert str(err.value) == "maximum height exceeds GPU limit"
+    huge = {"coefficients": ["1", "0", "1"], "bound": 2 ** 52, "first": 1, "last": 1}
+    small = {"coefficients": ["1", "0", "1"], "bound": 100, "first": 1, "last": 3}
+    with pytest.raises(pkg.SieveError) as err:
+        fresh_gpu.sieve_many([small, huge])
+    assert str(err.value) == "maximum height exceeds GPU limit"
 
 
 def test_curve_cache_switch(fresh_gpu, oracle_lib):
