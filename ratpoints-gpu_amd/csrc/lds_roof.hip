@@ -9,8 +9,13 @@
 // lane, one ds_read2st64_b64).
 // `lds_roof copy` instead runs one plain 256 MiB device-to-device copy kernel,
 // the reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters.
+// `lds_roof rows` times the wide-row dense stage's access pattern: one
+// workgroup of 16 waves per CU, every wave reading per iteration NL rows of
+// 16-byte elements from LDS (ds_read_b128 at (off_r + lane) * 16) and NG rows
+// from a per-workgroup global buffer that stays in the vector L1 cache
+// (global_load_dwordx4), row offsets advancing by 128 mod p_r per iteration.
 //
-// Usage: lds_roof [iterations] | lds_roof copy
+// Usage: lds_roof [iterations] | lds_roof copy | lds_roof rows [iterations]
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -117,11 +122,125 @@ int run_copy() {
     return 0;
 }
 
+constexpr int kRowSlot = 576;      // elements per row slot (p + 64 <= 573)
+constexpr int kRowThreads = 1024;  // 16 waves
+constexpr int kMaxRows = 14;
+__constant__ uint32_t c_row_p[kMaxRows] = {509, 503, 499, 491, 487, 479, 467,
+                                           463, 461, 457, 449, 443, 439, 433};
+
+template <int NL, int NG>
+__global__ __launch_bounds__(kRowThreads) void rows_kernel(int iterations, const uint4 *grows,
+                                                           unsigned *out) {
+    __shared__ __attribute__((aligned(16))) uint4 rows[(NL > 0 ? NL : 1) * kRowSlot];
+    for (int i = threadIdx.x; i < NL * kRowSlot; i += kRowThreads) {
+        rows[i] = make_uint4(0x9E3779B9u * (i + 1), 0x85EBCA6Bu * (i + 3),
+                             0xC2B2AE35u * (i + 5), 0x27D4EB2Fu * (i + 7));
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const uint4 *g = grows + static_cast<size_t>(blockIdx.x) * (NG > 0 ? NG : 1) * kRowSlot;
+    uint32_t off[NL + NG];
+    uint32_t p[NL + NG];
+#pragma unroll
+    for (int r = 0; r < NL + NG; ++r) {
+        p[r] = __builtin_amdgcn_readfirstlane(c_row_p[r]);
+        off[r] = __builtin_amdgcn_readfirstlane((wave * 131u + r * 37u) % p[r]);
+    }
+    uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (int it = 0; it < iterations; ++it) {
+        uint4 v[NL + NG];
+#pragma unroll
+        for (int r = 0; r < NL; ++r) {
+            v[r] = rows[r * kRowSlot + off[r] + lane];
+        }
+#pragma unroll
+        for (int r = 0; r < NG; ++r) {
+            v[NL + r] = g[r * kRowSlot + off[NL + r] + lane];
+        }
+#pragma unroll
+        for (int r = 0; r < NL + NG; ++r) {
+            const uint32_t next = off[r] + 128u;
+            off[r] = next >= p[r] ? next - p[r] : next;
+            asm volatile("" ::"s"(off[r]));
+        }
+#pragma unroll
+        for (int r = 0; r < NL + NG; ++r) {
+            a0 = __builtin_amdgcn_bitop3_b32(a0, v[r].x, v[r].y, 0x96);
+            a1 = __builtin_amdgcn_bitop3_b32(a1, v[r].z, v[r].w, 0x96);
+        }
+        a2 += a0;
+        a3 ^= a1;
+    }
+    if ((a0 ^ a1 ^ a2 ^ a3) == 0x1234567u) {
+        out[blockIdx.x] = a0;  // keeps the loads live; never true in practice
+    }
+}
+
+template <int NL, int NG>
+void measure_rows(int iterations, int blocks, const uint4 *grows, unsigned *out,
+                  double clk_hz) {
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    hipLaunchKernelGGL((rows_kernel<NL, NG>), dim3(blocks), dim3(kRowThreads), 0, 0, 10, grows,
+                       out);
+    (void)hipDeviceSynchronize();
+    (void)hipEventRecord(a);
+    hipLaunchKernelGGL((rows_kernel<NL, NG>), dim3(blocks), dim3(kRowThreads), 0, 0, iterations,
+                       grows, out);
+    (void)hipEventRecord(b);
+    (void)hipEventSynchronize(b);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    const double wave_iters_per_cu = double(kRowThreads / 64) * iterations;
+    const double clk = ms * 1e-3 * clk_hz;
+    std::printf("lds_rows=%d global_rows=%d ms=%.3f clk_per_wave_iteration_per_cu=%.1f "
+                "row_bytes_per_clk_per_cu=%.1f\n",
+                NL, NG, ms, clk / wave_iters_per_cu,
+                wave_iters_per_cu * (NL + NG) * 1024.0 / clk);
+    std::fflush(stdout);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+}
+
+int run_rows(int iterations) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+        return 1;
+    }
+    const int blocks = prop.multiProcessorCount;
+    const double clk_hz = prop.clockRate * 1e3;
+    const size_t n = static_cast<size_t>(blocks) * 3 * kRowSlot;
+    uint4 *grows = nullptr;
+    unsigned *out = nullptr;
+    if (hipMalloc(&grows, n * sizeof(uint4)) != hipSuccess
+        || hipMalloc(&out, blocks * sizeof(unsigned)) != hipSuccess) {
+        return 1;
+    }
+    (void)hipMemset(grows, 0x5A, n * sizeof(uint4));
+    (void)hipDeviceSynchronize();
+    std::printf("cus=%d clock_khz=%d iterations=%d\n", blocks, prop.clockRate, iterations);
+    measure_rows<14, 0>(iterations, blocks, grows, out, clk_hz);  // warm-up pass
+    measure_rows<14, 0>(iterations, blocks, grows, out, clk_hz);
+    measure_rows<12, 0>(iterations, blocks, grows, out, clk_hz);
+    measure_rows<12, 2>(iterations, blocks, grows, out, clk_hz);
+    measure_rows<11, 3>(iterations, blocks, grows, out, clk_hz);
+    measure_rows<0, 2>(iterations, blocks, grows, out, clk_hz);
+    measure_rows<0, 3>(iterations, blocks, grows, out, clk_hz);
+    (void)hipFree(grows);
+    (void)hipFree(out);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
     if (argc > 1 && std::string(argv[1]) == "copy") {
         return run_copy();
+    }
+    if (argc > 1 && std::string(argv[1]) == "rows") {
+        return run_rows(argc > 2 ? std::atoi(argv[2]) : 20000);
     }
     const int iterations = argc > 1 ? std::atoi(argv[1]) : 20000;
     hipDeviceProp_t prop;
