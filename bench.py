@@ -14,7 +14,9 @@ production.  Inputs (the curve's residue table) are resident before timing.
 Printed: one JSON line.  ``value`` is Gsite/s over the timed batches
 (kernels + survivor download + sink); ``roofline`` uses the sieve kernel's own
 HIP-event time; ``cpu_baseline`` times the CPU oracle's packed-word sieve on a
-bounded sample of the same workload on the host cores.
+bounded sample of the same workload on the host cores, and ``sample_parity``
+checks the timed run's survivors on that sample (b = 1..n) against the
+oracle's: same count, same order-independent digest.
 """
 
 import argparse
@@ -49,9 +51,13 @@ def load(name, path):
     return module
 
 
-def cpu_baseline(coeffs, seconds_target=15.0):
+def cpu_baseline(coeffs, gpu_num, gpu_den, gpu_last, seconds_target=15.0):
     """The oracle's packed-word CPU sieve on the same curve and A, over the
-    first n denominators, n sized for about `seconds_target` of wall time."""
+    first n denominators, n sized for about `seconds_target` of wall time.
+    The oracle's survivors of that sample are also the checker for the timed
+    GPU run over b = 1..gpu_last (n <= gpu_last): its survivors with b <= n
+    must have the same count and digest.
+    Returns (baseline, sample_parity)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     lib = oracle.OracleLib()
@@ -61,13 +67,13 @@ def cpu_baseline(coeffs, seconds_target=15.0):
     lib.sieve(coeffs, BOUND, 1, n, method="words", threads=threads, collect=False)
     dt = time.perf_counter() - t0
     n = max(threads, int(n * seconds_target / max(dt, 1e-3)))
-    n = min(n, LAST)
+    n = min(n, LAST, gpu_last)
     t0 = time.perf_counter()
-    count, _, _, _ = lib.sieve(coeffs, BOUND, 1, n, method="words", threads=threads,
-                               collect=False)
+    count, digest, _, _ = lib.sieve(coeffs, BOUND, 1, n, method="words", threads=threads,
+                                    collect=False)
     dt = time.perf_counter() - t0
     sites = (2 * BOUND + 1) * n
-    return {
+    baseline = {
         "value": round(sites / dt / 1e9, 3),
         "unit": "Gsite/s",
         "cores": threads,
@@ -75,6 +81,16 @@ def cpu_baseline(coeffs, seconds_target=15.0):
         "sample": f"oracle packed-word sieve, same curve, A=5e6, b=1..{n} "
                   f"({sites:.3e} sites, {dt:.1f} s, {count} survivors)",
     }
+    sel = gpu_den <= n
+    gpu_count = int(sel.sum())
+    parity = {
+        "denominators": f"1..{n}",
+        "oracle_survivors": int(count),
+        "gpu_survivors": gpu_count,
+        "identical": gpu_count == count
+        and oracle.site_digest(gpu_num[sel], gpu_den[sel]) == tuple(digest),
+    }
+    return baseline, parity
 
 
 def measured_traffic(variant):
@@ -392,7 +408,9 @@ def main():
             result["error"] = "served results differ from the searches alone"
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline(coeffs)
+        result["cpu_baseline"], result["sample_parity"] = cpu_baseline(scoeffs, num, den, last_t)
+        if not result["sample_parity"]["identical"]:
+            result["error"] = "timed GPU survivors differ from the oracle on the CPU sample"
     print(json.dumps(result), flush=True)
     if "error" in result:
         sys.exit(1)
