@@ -378,13 +378,43 @@ struct RowTile {
         }
     }
 
+    // Row offsets of a wave: lane s < FAST holds dense prime s and the
+    // offset W mod p_s of the wave's next iteration; the other lanes hold 0
+    // and p = kStride, so their offset stays 0.
+    struct Offsets {
+        uint32_t off;
+        uint32_t p;
+    };
+
+    // Lane s < FAST gets prime s and the offset of word `rel` (from the tile
+    // start) in its row.
+    __device__ static Offsets row_offsets(const uint32_t *s_p, const uint32_t *s_base,
+                                          const uint32_t *s_magic, uint32_t rel, int lane) {
+        Offsets o{0u, static_cast<uint32_t>(kStride)};
+        if (lane < FAST) {
+            const uint32_t p = s_p[lane];
+            const uint32_t v = mod_small(rel, p, s_magic[lane]) + s_base[lane];
+            o.off = v >= p ? v - p : v;
+            o.p = p;
+        }
+        return o;
+    }
+
     // Dense stage of one iteration: m[u] = AND over the FAST rows of entry
-    // off[s] + lane + 64 u, then every offset advances one stride (mod p).
+    // off_s + lane + 64 u, then every offset advances one stride (mod p).
     // Rows are combined two at a time on 32-bit halves with the three-input
-    // AND (v_bitop3_b32), one instruction per half for two rows.
-    __device__ static void dense(const uint64_t *pool, uint32_t (&off)[FAST],
-                                 const uint32_t (&prime)[FAST], int lane,
+    // AND (v_bitop3_b32), one instruction per half for two rows.  The
+    // offsets live in one VGPR (lane s = row s): read back with v_readlane
+    // and advanced by three vector instructions for all rows, instead of
+    // three scalar instructions per row, which made the scalar unit the
+    // limit of the loop.
+    __device__ static void dense(const uint64_t *pool, Offsets &o, int lane,
                                  uint64_t (&m)[U]) {
+        uint32_t off[FAST];
+#pragma unroll
+        for (int s = 0; s < FAST; ++s) {
+            off[s] = __builtin_amdgcn_readlane(o.off, s);
+        }
         // All loads first, so that each wave keeps FAST reads in flight.
         uint64_t x[FAST][U];
 #pragma unroll
@@ -401,14 +431,9 @@ struct RowTile {
                 }
             }
         }
-        // The scalar offset updates run while the reads are in flight: the
-        // empty asm takes the new offsets as SGPR operands here, so they are
-        // not sunk behind the AND and the queue branch into the loop latch.
-#pragma unroll
-        for (int s = 0; s < FAST; ++s) {
-            const uint32_t next = off[s] + kStride;
-            off[s] = next >= prime[s] ? next - prime[s] : next;
-            asm volatile("" ::"s"(off[s]));
+        {
+            const uint32_t next = o.off + kStride;
+            o.off = next >= o.p ? next - o.p : next;
         }
         uint32_t lo[U], hi[U];
 #pragma unroll
@@ -450,23 +475,8 @@ struct RowTile {
         const uint32_t *s_base = s_c + 3 * kSelected;
         const uint32_t *s_magic = s_base + kSelected;
         const uint64_t *cext = pool + kCext;
-        // Wave-uniform row offsets: word ws mod p for each dense prime.
-        uint32_t off[FAST];
-        uint32_t prime[FAST];
-        {
-            const uint32_t rel = static_cast<uint32_t>(ws - ts);
-            uint32_t mine = 0;
-            if (lane < FAST) {
-                const uint32_t p = s_p[lane];
-                const uint32_t v = mod_small(rel, p, s_magic[lane]) + s_base[lane];
-                mine = v >= p ? v - p : v;
-            }
-#pragma unroll
-            for (int s = 0; s < FAST; ++s) {
-                off[s] = __builtin_amdgcn_readlane(mine, s);
-                prime[s] = uniform(s_p[s]);
-            }
-        }
+        // Row offsets: word ws mod p for each dense prime.
+        Offsets o = row_offsets(s_p, s_base, s_magic, static_cast<uint32_t>(ws - ts), lane);
         // Late-prime constants: lane j of each 32-lane half owns prime
         // FAST + j.
         const int lj = lane & 31;
@@ -485,7 +495,7 @@ struct RowTile {
         uint64_t W = ws;
         for (; W < we; W += kStride) {
             uint64_t m[U];
-            dense(pool, off, prime, lane, m);
+            dense(pool, o, lane, m);
             if (W + kStride >= we) {
                 // Last iteration of this wave: drop words past the wave's
                 // range and the sites past +A in the line's final word.
@@ -686,22 +696,8 @@ struct RowTileQ {
         // offset from the tile start modulo the stride.
         const uint32_t skip0 = static_cast<uint32_t>((from - ts) & (kStride - 1));
         const uint64_t ws = from - skip0;
-        uint32_t off[FAST];
-        uint32_t prime[FAST];
-        {
-            const uint32_t rel = static_cast<uint32_t>(ws - ts);
-            uint32_t mine = 0;
-            if (lane < FAST) {
-                const uint32_t p = s_p[lane];
-                const uint32_t v = mod_small(rel, p, s_magic[lane]) + s_base[lane];
-                mine = v >= p ? v - p : v;
-            }
-#pragma unroll
-            for (int s = 0; s < FAST; ++s) {
-                off[s] = __builtin_amdgcn_readlane(mine, s);
-                prime[s] = uniform(s_p[s]);
-            }
-        }
+        typename Base::Offsets o =
+            Base::row_offsets(s_p, s_base, s_magic, static_cast<uint32_t>(ws - ts), lane);
         // Iterations whose words all lie below min(we, last word) need no
         // masking; they run the unmasked body.  The first iteration of a
         // resumed sweep and the range's tail run the masked one.
@@ -717,21 +713,21 @@ struct RowTileQ {
         const uint32_t rel0 = static_cast<uint32_t>(ws - ts);
         uint32_t it = 0;
         if (skip0 && n_iter) {
-            if (!step<true>(pool, ws, rel0, we, last_word, skip0, off, prime, q, late, b, ts, L,
+            if (!step<true>(pool, ws, rel0, we, last_word, skip0, o, q, late, b, ts, L,
                             O, request, lane)) {
                 return oldest(q, ts);
             }
             it = 1;
         }
         for (; it < n_clean; ++it) {
-            if (!step<false>(pool, 0, rel0 + it * kStride, we, last_word, 0u, off, prime, q,
+            if (!step<false>(pool, 0, rel0 + it * kStride, we, last_word, 0u, o, q,
                              late, b, ts, L, O, request, lane)) {
                 return oldest(q, ts);
             }
         }
         for (; it < n_iter; ++it) {
             if (!step<true>(pool, ws + static_cast<uint64_t>(it) * kStride, rel0 + it * kStride,
-                            we, last_word, 0u, off, prime, q, late, b, ts, L, O, request,
+                            we, last_word, 0u, o, q, late, b, ts, L, O, request,
                             lane)) {
                 return oldest(q, ts);
             }
@@ -769,12 +765,12 @@ struct RowTileQ {
     // W is read only by the EDGE body; rel_w = W - ts.
     template <bool EDGE>
     __device__ static bool step(uint64_t *pool, uint64_t W, uint32_t rel_w, uint64_t we,
-                                uint64_t last_word, uint32_t skip, uint32_t (&off)[FAST],
-                                const uint32_t (&prime)[FAST], Queue &q, const Late &late,
+                                uint64_t last_word, uint32_t skip,
+                                typename Base::Offsets &o, Queue &q, const Late &late,
                                 int b, uint64_t ts, const LineDesc &L, const OutDesc &O,
                                 int request, int lane) {
         uint64_t m[U];
-        Base::dense(pool, off, prime, lane, m);
+        Base::dense(pool, o, lane, m);
         if (EDGE) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
