@@ -13,7 +13,10 @@
 // workgroup of 16 waves per CU, every wave reading per iteration NL rows of
 // 16-byte elements from LDS (ds_read_b128 at (off_r + lane) * 16) and NG rows
 // from a per-workgroup global buffer that stays in the vector L1 cache
-// (global_load_dwordx4), row offsets advancing by 128 mod p_r per iteration.
+// (global_load_dwordx4), row offsets advancing by 128 mod p_r per iteration:
+// three scalar instructions per row, or (offsets=vgpr) one VGPR holding
+// every row's offset, advanced by three vector instructions and read back
+// with v_readlane.
 //
 // Usage: lds_roof [iterations] | lds_roof copy | lds_roof rows [iterations]
 #include <hip/hip_runtime.h>
@@ -128,9 +131,10 @@ constexpr int kMaxRows = 14;
 __constant__ uint32_t c_row_p[kMaxRows] = {509, 503, 499, 491, 487, 479, 467,
                                            463, 461, 457, 449, 443, 439, 433};
 
-template <int NL, int NG>
+template <int NL, int NG, bool VOFF = false>
 __global__ __launch_bounds__(kRowThreads) void rows_kernel(int iterations, const uint4 *grows,
                                                            unsigned *out) {
+    static_assert(!VOFF || NG == 0, "vector offsets are probed on LDS rows only");
     __shared__ __attribute__((aligned(16))) uint4 rows[(NL > 0 ? NL : 1) * kRowSlot];
     for (int i = threadIdx.x; i < NL * kRowSlot; i += kRowThreads) {
         rows[i] = make_uint4(0x9E3779B9u * (i + 1), 0x85EBCA6Bu * (i + 3),
@@ -147,9 +151,18 @@ __global__ __launch_bounds__(kRowThreads) void rows_kernel(int iterations, const
         p[r] = __builtin_amdgcn_readfirstlane(c_row_p[r]);
         off[r] = __builtin_amdgcn_readfirstlane((wave * 131u + r * 37u) % p[r]);
     }
+    // Offsets=vgpr: lane r < NL holds row r's offset and prime.
+    const uint32_t lp = lane < NL ? c_row_p[lane] : 128u;
+    uint32_t voff = lane < NL ? (wave * 131u + lane * 37u) % lp : 0u;
     uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
     for (int it = 0; it < iterations; ++it) {
         uint4 v[NL + NG];
+        if (VOFF) {
+#pragma unroll
+            for (int r = 0; r < NL; ++r) {
+                off[r] = __builtin_amdgcn_readlane(voff, r);
+            }
+        }
 #pragma unroll
         for (int r = 0; r < NL; ++r) {
             v[r] = rows[r * kRowSlot + off[r] + lane];
@@ -158,11 +171,16 @@ __global__ __launch_bounds__(kRowThreads) void rows_kernel(int iterations, const
         for (int r = 0; r < NG; ++r) {
             v[NL + r] = g[r * kRowSlot + off[NL + r] + lane];
         }
+        if (VOFF) {
+            const uint32_t next = voff + 128u;
+            voff = next >= lp ? next - lp : next;
+        } else {
 #pragma unroll
-        for (int r = 0; r < NL + NG; ++r) {
-            const uint32_t next = off[r] + 128u;
-            off[r] = next >= p[r] ? next - p[r] : next;
-            asm volatile("" ::"s"(off[r]));
+            for (int r = 0; r < NL + NG; ++r) {
+                const uint32_t next = off[r] + 128u;
+                off[r] = next >= p[r] ? next - p[r] : next;
+                asm volatile("" ::"s"(off[r]));
+            }
         }
 #pragma unroll
         for (int r = 0; r < NL + NG; ++r) {
@@ -177,27 +195,27 @@ __global__ __launch_bounds__(kRowThreads) void rows_kernel(int iterations, const
     }
 }
 
-template <int NL, int NG>
+template <int NL, int NG, bool VOFF = false>
 void measure_rows(int iterations, int blocks, const uint4 *grows, unsigned *out,
                   double clk_hz) {
     hipEvent_t a, b;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
-    hipLaunchKernelGGL((rows_kernel<NL, NG>), dim3(blocks), dim3(kRowThreads), 0, 0, 10, grows,
-                       out);
+    hipLaunchKernelGGL((rows_kernel<NL, NG, VOFF>), dim3(blocks), dim3(kRowThreads), 0, 0, 10,
+                       grows, out);
     (void)hipDeviceSynchronize();
     (void)hipEventRecord(a);
-    hipLaunchKernelGGL((rows_kernel<NL, NG>), dim3(blocks), dim3(kRowThreads), 0, 0, iterations,
-                       grows, out);
+    hipLaunchKernelGGL((rows_kernel<NL, NG, VOFF>), dim3(blocks), dim3(kRowThreads), 0, 0,
+                       iterations, grows, out);
     (void)hipEventRecord(b);
     (void)hipEventSynchronize(b);
     float ms = 0;
     (void)hipEventElapsedTime(&ms, a, b);
     const double wave_iters_per_cu = double(kRowThreads / 64) * iterations;
     const double clk = ms * 1e-3 * clk_hz;
-    std::printf("lds_rows=%d global_rows=%d ms=%.3f clk_per_wave_iteration_per_cu=%.1f "
-                "row_bytes_per_clk_per_cu=%.1f\n",
-                NL, NG, ms, clk / wave_iters_per_cu,
+    std::printf("lds_rows=%d global_rows=%d offsets=%s ms=%.3f "
+                "clk_per_wave_iteration_per_cu=%.1f row_bytes_per_clk_per_cu=%.1f\n",
+                NL, NG, VOFF ? "vgpr" : "sgpr", ms, clk / wave_iters_per_cu,
                 wave_iters_per_cu * (NL + NG) * 1024.0 / clk);
     std::fflush(stdout);
     (void)hipEventDestroy(a);
@@ -228,6 +246,10 @@ int run_rows(int iterations) {
     measure_rows<11, 3>(iterations, blocks, grows, out, clk_hz);
     measure_rows<0, 2>(iterations, blocks, grows, out, clk_hz);
     measure_rows<0, 3>(iterations, blocks, grows, out, clk_hz);
+    measure_rows<14, 0, true>(iterations, blocks, grows, out, clk_hz);
+    measure_rows<12, 0, true>(iterations, blocks, grows, out, clk_hz);
+    measure_rows<14, 0>(iterations, blocks, grows, out, clk_hz);
+    measure_rows<14, 0, true>(iterations, blocks, grows, out, clk_hz);
     (void)hipFree(grows);
     (void)hipFree(out);
     return 0;
