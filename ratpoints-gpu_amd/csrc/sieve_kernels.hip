@@ -380,22 +380,38 @@ struct RowTile {
 
     // Row offsets of a wave: lane s < FAST holds dense prime s and the
     // offset W mod p_s of the wave's next iteration; the other lanes hold 0
-    // and p = kStride, so their offset stays 0.
+    // and p = kStride, so their offset stays 0.  The first kScalarRows rows
+    // of the wide tile also keep theirs in SGPRs, advanced by three scalar
+    // instructions each: that balances the scalar unit (idle with every
+    // offset in the VGPR) against the vector pipes (busiest with it).
+    // Measured on 156,251-word lines: 0 rows 53.8-53.9, 3 rows 54.8-55.0,
+    // 4-6 rows 55.1-55.3, 7 rows 54.6, all 14 rows 51.6-51.8 Tsite/s; the
+    // narrow tiles gain nothing from it.
+    static constexpr int kScalarRows = WIDE ? (FAST < 5 ? FAST : 5) : 0;
     struct Offsets {
         uint32_t off;
         uint32_t p;
+        uint32_t soff[kScalarRows > 0 ? kScalarRows : 1];
+        uint32_t sp[kScalarRows > 0 ? kScalarRows : 1];
     };
 
     // Lane s < FAST gets prime s and the offset of word `rel` (from the tile
     // start) in its row.
     __device__ static Offsets row_offsets(const uint32_t *s_p, const uint32_t *s_base,
                                           const uint32_t *s_magic, uint32_t rel, int lane) {
-        Offsets o{0u, static_cast<uint32_t>(kStride)};
+        Offsets o{};
+        o.off = 0u;
+        o.p = static_cast<uint32_t>(kStride);
         if (lane < FAST) {
             const uint32_t p = s_p[lane];
             const uint32_t v = mod_small(rel, p, s_magic[lane]) + s_base[lane];
             o.off = v >= p ? v - p : v;
             o.p = p;
+        }
+#pragma unroll
+        for (int s = 0; s < kScalarRows; ++s) {
+            o.soff[s] = __builtin_amdgcn_readlane(o.off, s);
+            o.sp[s] = __builtin_amdgcn_readlane(o.p, s);
         }
         return o;
     }
@@ -407,13 +423,14 @@ struct RowTile {
     // offsets live in one VGPR (lane s = row s): read back with v_readlane
     // and advanced by three vector instructions for all rows, instead of
     // three scalar instructions per row, which made the scalar unit the
-    // limit of the loop.
+    // limit of the loop; the first kScalarRows rows use their SGPR copies.
     __device__ static void dense(const uint64_t *pool, Offsets &o, int lane,
                                  uint64_t (&m)[U]) {
         uint32_t off[FAST];
 #pragma unroll
         for (int s = 0; s < FAST; ++s) {
-            off[s] = __builtin_amdgcn_readlane(o.off, s);
+            off[s] = s < kScalarRows ? o.soff[s < kScalarRows ? s : 0]
+                                     : __builtin_amdgcn_readlane(o.off, s);
         }
         // All loads first, so that each wave keeps FAST reads in flight.
         uint64_t x[FAST][U];
@@ -434,6 +451,15 @@ struct RowTile {
         {
             const uint32_t next = o.off + kStride;
             o.off = next >= o.p ? next - o.p : next;
+        }
+        // The scalar copies advance while the reads are in flight (the empty
+        // asm pins the new values in SGPRs here, ahead of the AND and the
+        // queue branch).
+#pragma unroll
+        for (int s = 0; s < kScalarRows; ++s) {
+            const uint32_t next = o.soff[s] + kStride;
+            o.soff[s] = next >= o.sp[s] ? next - o.sp[s] : next;
+            asm volatile("" ::"s"(o.soff[s]));
         }
         uint32_t lo[U], hi[U];
 #pragma unroll
