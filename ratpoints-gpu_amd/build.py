@@ -119,15 +119,6 @@ def build_serve(backend=None, out=None):
     return build_cli(backend, out, SERVE_SOURCES, "ratpoints_serve")
 
 
-def build_probe():
-    """bin/lds_roof: the LDS read-bandwidth probe behind the roofline peak."""
-    os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
-    out = os.path.join(HERE, "bin", "lds_roof")
-    run([HIPCC, *HIP_FLAGS, os.path.join(HERE, "csrc", "lds_roof.hip"), "-o", out + ".tmp"])
-    os.replace(out + ".tmp", out)
-    return out
-
-
 def build_oracle():
     """The oracle is test infrastructure; it is built beside the product so
     tests, smoke() and the bench's CPU baseline find it."""
@@ -155,7 +146,6 @@ def build_all():
     lib = build_library()
     cli = build_cli()
     build_serve()
-    build_probe()
     build_oracle()
     return lib, cli
 
