@@ -86,6 +86,14 @@ def load_library(path=LIB_PATH):
     if not os.path.exists(path):
         raise RuntimeError(
             f"HIP sieve library not built: {path}; run __graft_entry__.build()")
+    _LIB = bind_library(path)
+    return _LIB
+
+
+def bind_library(path):
+    """Declares the C ABI's prototypes on the shared library at `path`
+    (another build of the same ABI, for tests) without making it the
+    module's default library."""
     lib = ctypes.CDLL(path)
     lib.rps_open.restype = ctypes.c_int
     lib.rps_open.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
@@ -116,7 +124,6 @@ def load_library(path=LIB_PATH):
     lib.rps_last_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_uint64),
                                            ctypes.POINTER(ctypes.c_double)]
-    _LIB = lib
     return lib
 
 
@@ -156,8 +163,8 @@ class SieveResult:
 class SieveContext:
     """One device context (rps_open/rps_close)."""
 
-    def __init__(self, device=0, **options):
-        self.lib = load_library()
+    def __init__(self, device=0, lib=None, **options):
+        self.lib = lib if lib is not None else load_library()
         handle = ctypes.c_void_p()
         rc = self.lib.rps_open(device, ctypes.byref(handle))
         if rc != RPS_OK:

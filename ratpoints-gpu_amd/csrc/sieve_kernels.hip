@@ -22,6 +22,8 @@
 
 #include <cstdint>
 
+#include <gfx950_asm.hpp>
+
 #include "sieve_kernels.hpp"
 #include "sieve_layout.hpp"
 
@@ -452,14 +454,14 @@ struct RowTile {
             const uint32_t next = o.off + kStride;
             o.off = next >= o.p ? next - o.p : next;
         }
-        // The scalar copies advance while the reads are in flight (the empty
-        // asm pins the new values in SGPRs here, ahead of the AND and the
-        // queue branch).
+        // The scalar copies advance while the reads are in flight
+        // (keep_in_sgpr pins the new values in SGPRs here, ahead of the AND
+        // and the queue branch).
 #pragma unroll
         for (int s = 0; s < kScalarRows; ++s) {
             const uint32_t next = o.soff[s] + kStride;
             o.soff[s] = next >= o.sp[s] ? next - o.sp[s] : next;
-            asm volatile("" ::"s"(o.soff[s]));
+            keep_in_sgpr(o.soff[s]);
         }
         uint32_t lo[U], hi[U];
 #pragma unroll
@@ -1258,7 +1260,7 @@ __global__ void reset_state_kernel(DeviceState *state) {
 // a cursor (line of the run, position) so that a launch after a full buffer
 // resumes exactly where each wave stopped.
 template <class Tile, int WAVES>
-__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4))) void sieve_kernel(
+__global__ __launch_bounds__(WAVES * 64, 4) void sieve_kernel(
     const SieveParams P) {
     using Run = RowRun<Tile>;
     __shared__ __attribute__((aligned(16))) uint64_t pool[Run::kLdsWords];
@@ -1402,7 +1404,7 @@ __global__ __launch_bounds__(WAVES * 64) void direct_kernel(const SieveParams P)
 // host chose; bucket[g] is the curve holding tile g << bucket_shift, so a
 // tile's curve is found by a binary search over a few curves at most.
 template <class Row, int WAVES>
-__global__ __launch_bounds__(WAVES * 64) __attribute__((amdgpu_waves_per_eu(4))) void multi_kernel(
+__global__ __launch_bounds__(WAVES * 64, 4) void multi_kernel(
     const MultiParams M) {
     using Run = RowRun<Row>;
     using Direct = DirectTile<WAVES, kDirectMaxDenominators>;

@@ -31,6 +31,15 @@ def load_package():
     return _load("ratpoints_gpu_amd", os.path.join(ROOT, "ratpoints-gpu_amd", "__init__.py"))
 
 
+def compiled_variants():
+    """(index, name) of every row-tile variant compiled into the library, read
+    from its C ABI (no device needed); empty when it has not been built."""
+    try:
+        return list(enumerate(load_package().variants()))
+    except Exception:
+        return []
+
+
 def load_workloads():
     return _load("ratpoints_gpu_amd_workloads",
                  os.path.join(ROOT, "ratpoints-gpu_amd", "workloads.py"))

@@ -12,10 +12,15 @@ import random
 import numpy as np
 import pytest
 
+import conftest
+
 import curves
 import oracle
 
 pytestmark = pytest.mark.gpu
+
+VARIANTS = [i for i, _ in conftest.compiled_variants()]
+VARIANT_IDS = [name for _, name in conftest.compiled_variants()]
 
 
 def announce(**case):
@@ -82,12 +87,10 @@ def test_many_long_lines_and_chunks(fresh_gpu):
         assert np.array_equal(num, pn) and np.array_equal(den, pd)
 
 
-@pytest.mark.parametrize("variant", range(16))
+@pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
 def test_many_overflow_requeues_exactly(fresh_gpu, oracle_lib, pkg, variant):
     # Dense requests overflow a 16,384-survivor buffer: they are searched
     # again alone; every request still matches the oracle.
-    if variant >= len(pkg.variants()):
-        pytest.skip("variant not compiled")
     fresh_gpu.set_option("variant", variant)
     fresh_gpu.set_option("survivor_capacity", 16384)
     dense = ["1", str(curves.dense_product())]

@@ -15,10 +15,15 @@ import random
 import numpy as np
 import pytest
 
+import conftest
+
 import curves
 import oracle
 
 pytestmark = pytest.mark.gpu
+
+VARIANTS = [i for i, _ in conftest.compiled_variants()]
+VARIANT_IDS = [name for _, name in conftest.compiled_variants()]
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 ROWS = {r["name"]: r for r in json.load(open(os.path.join(GOLDEN, "reference_survivor_counts.json")))}
@@ -155,10 +160,8 @@ def test_divisible_denominators(fresh_gpu, oracle_lib):
         assert_same(res, oracle_lib.sieve(coeffs, 30000, b, b + 2, method="words"))
 
 
-@pytest.mark.parametrize("variant", range(16))
+@pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
 def test_variants_agree(fresh_gpu, oracle_lib, pkg, variant):
-    if variant >= len(pkg.variants()):
-        pytest.skip("variant not compiled")
     coeffs, _ = curves.partially_dense(random.Random(77), 7, 6)
     announce(case="variant", variant=pkg.variants()[variant])
     fresh_gpu.set_option("variant", variant)
@@ -166,13 +169,11 @@ def test_variants_agree(fresh_gpu, oracle_lib, pkg, variant):
     assert_same(res, oracle_lib.sieve(coeffs, 70001, 11, 140, method="words"))
 
 
-@pytest.mark.parametrize("variant", range(16))
+@pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
 def test_variants_resume_exactly_once(fresh_gpu, oracle_lib, pkg, variant):
     # About a quarter of all sites survive (two live primes), so every wave
     # overflows a 16,384-survivor buffer many times and resumes from its
     # cursor; the multiset must still be exact.
-    if variant >= len(pkg.variants()):
-        pytest.skip("variant not compiled")
     coeffs, _ = curves.partially_dense(random.Random(78 + variant), 2, 4)
     announce(case="variant-resume", variant=pkg.variants()[variant], capacity=16384)
     fresh_gpu.set_option("variant", variant)
@@ -182,13 +183,11 @@ def test_variants_resume_exactly_once(fresh_gpu, oracle_lib, pkg, variant):
     assert_same(res, oracle_lib.sieve(coeffs, 40000, 3, 22, method="words"))
 
 
-@pytest.mark.parametrize("variant", range(16))
+@pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
 def test_runs_of_lines(fresh_gpu, oracle_lib, pkg, variant):
     # min_tiles = 1 lets lines shorter than 2^17 words share a tile: 2,501-word
     # lines go 52 to a tile, so b = 7..140 is two full runs and a partial
     # one, each line's strings loaded while the previous line is swept.
-    if variant >= len(pkg.variants()):
-        pytest.skip("variant not compiled")
     coeffs, _ = curves.partially_dense(random.Random(90 + variant), 7, 6)
     announce(case="runs", variant=pkg.variants()[variant])
     fresh_gpu.set_option("variant", variant)
@@ -202,13 +201,11 @@ def test_runs_of_lines(fresh_gpu, oracle_lib, pkg, variant):
     assert_same(res, oracle_lib.sieve(coeffs, 3000, 1, 200, square=True, method="words"))
 
 
-@pytest.mark.parametrize("variant", range(16))
+@pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
 def test_runs_resume_exactly_once(fresh_gpu, oracle_lib, pkg, variant):
     # Runs of 104 lines of 1,251 words with about a quarter of all sites
     # surviving: waves stop mid-run on a full 16,384-survivor buffer and
     # resume at their (line, word) cursors.
-    if variant >= len(pkg.variants()):
-        pytest.skip("variant not compiled")
     coeffs, _ = curves.partially_dense(random.Random(120 + variant), 2, 5)
     announce(case="runs-resume", variant=pkg.variants()[variant], capacity=16384)
     fresh_gpu.set_option("variant", variant)
