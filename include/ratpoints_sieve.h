@@ -128,6 +128,24 @@ typedef int (*rps_batch_done)(void *user, int32_t first_t, int32_t last_t,
 int rps_sieve_batched(rps_ctx *ctx, const rps_request *request, rps_sink sink,
                       rps_batch_done done, void *user, rps_stats *stats);
 
+/* Allocation-free form of rps_sieve.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory while they run (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The call makes no device or host allocation: the survivors go to the
+ * caller's device buffers `numerators` and `denominators` (`capacity`
+ * entries each, any order), and the kernels' partial results (the curve's
+ * residue table, flush state, cursors) live in a fixed device scratch array
+ * of 3.4 MB per device, held until the process ends, whose address rps_open
+ * resolves.  Limits: lines of at most 2^39 64-bit words (A < 1.7e13), else
+ * RPS_EINVAL; at most 16,384 tiles per launch, which the call's batching
+ * keeps to.  Returns after the survivors are written, with
+ * *survivor_count set.  When they exceed `capacity` the call returns
+ * RPS_EOVERFLOW and *survivor_count survivors of the request (a subset)
+ * are in the buffers.  Every call on one device shares its scratch array:
+ * calls that overlap on one GPU (from two threads or contexts) race. */
+int rps_sieve_into(rps_ctx *ctx, const rps_request *request, int64_t *numerators,
+                   int32_t *denominators, uint64_t capacity, uint64_t *survivor_count);
+
 /* Checks a request without touching a device: RPS_OK, or RPS_EINVAL with the
  * reason rps_sieve would give written to `message` (NUL-terminated, at most
  * message_len bytes; may be NULL).  No counterpart in the reference, whose

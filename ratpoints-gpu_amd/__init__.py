@@ -107,6 +107,10 @@ def bind_library(path):
     lib.rps_sieve_many.restype = ctypes.c_int
     lib.rps_sieve_many.argtypes = [ctypes.c_void_p, ctypes.POINTER(RpsRequest), ctypes.c_size_t,
                                    MULTI_SINK, ctypes.c_void_p, ctypes.POINTER(RpsStats)]
+    lib.rps_sieve_into.restype = ctypes.c_int
+    lib.rps_sieve_into.argtypes = [ctypes.c_void_p, ctypes.POINTER(RpsRequest), ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_uint64,
+                                   ctypes.POINTER(ctypes.c_uint64)]
     lib.rps_set_option.restype = ctypes.c_int
     lib.rps_set_option.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]
     lib.rps_variant_count.restype = ctypes.c_int
@@ -232,6 +236,24 @@ class SieveContext:
         if rc not in (RPS_OK, RPS_ECANCELLED):
             raise SieveError(rc, self.error())
         return rc, stats.as_dict()
+
+    def sieve_into(self, coefficients, bound, first, last, numerators, denominators,
+                   capacity, square=False):
+        """rps_sieve_into: survivors written to the device buffers at the
+        addresses `numerators` (int64) and `denominators` (int32) of
+        `capacity` entries each.  Returns (status, survivor count); raises on
+        errors other than RPS_EOVERFLOW."""
+        coeffs = [str(c) for c in coefficients]
+        arr = (ctypes.c_char_p * len(coeffs))()
+        arr[:] = [c.encode() for c in coeffs]
+        req = RpsRequest(arr, len(coeffs), int(bound), int(first), int(last), int(bool(square)))
+        count = ctypes.c_uint64()
+        rc = self.lib.rps_sieve_into(self.handle, ctypes.byref(req), ctypes.c_void_p(numerators),
+                                     ctypes.c_void_p(denominators), int(capacity),
+                                     ctypes.byref(count))
+        if rc not in (RPS_OK, RPS_EOVERFLOW):
+            raise SieveError(rc, self.error())
+        return rc, count.value
 
     def run_many(self, requests, sink=None):
         """Runs rps_sieve_many.  ``requests`` is a list of dicts with keys

@@ -173,6 +173,7 @@ struct rps_ctx {
     unsigned long long *d_resid = nullptr;
     int *d_negmod = nullptr;
     int *h_negmod = nullptr;  // pinned
+    int *h_coef = nullptr;    // pinned [36][101]: coefficient residues for rps_sieve_into
     std::vector<std::string> curve_key;
     bool curve_ready = false;
     Slot slots[2];
@@ -469,6 +470,8 @@ int rps_open(int device, rps_ctx **out) {
         }
         ctx->device = device;
         RPS_HIP(hipSetDevice(device));
+        // The allocation-free entry point's scratch address on this device.
+        RPS_HIP(rps::scratch_init(device));
         hipDeviceProp_t prop;
         RPS_HIP(hipGetDeviceProperties(&prop, device));
         ctx->cu_count = prop.multiProcessorCount;
@@ -485,6 +488,8 @@ int rps_open(int device, rps_ctx **out) {
         RPS_HIP(hipMalloc(&ctx->d_resid, rps::kResidTableWords * sizeof(unsigned long long)));
         RPS_HIP(hipMalloc(&ctx->d_negmod, rps::kCandidates * sizeof(int)));
         RPS_HIP(hipHostMalloc(&ctx->h_negmod, rps::kCandidates * sizeof(int)));
+        RPS_HIP(hipHostMalloc(&ctx->h_coef, static_cast<size_t>(rps::kCandidates)
+                                  * rps::kMaxCoefficients * sizeof(int)));
         RPS_HIP(hipEventCreate(&ctx->basis_start));
         RPS_HIP(hipEventCreate(&ctx->basis_end));
         for (Slot &s : ctx->slots) {
@@ -526,6 +531,7 @@ void rps_close(rps_ctx *ctx) {
     if (ctx->d_resid) (void)hipFree(ctx->d_resid);
     if (ctx->d_negmod) (void)hipFree(ctx->d_negmod);
     if (ctx->h_negmod) (void)hipHostFree(ctx->h_negmod);
+    if (ctx->h_coef) (void)hipHostFree(ctx->h_coef);
     if (ctx->h_num) (void)hipHostFree(ctx->h_num);
     if (ctx->h_den) (void)hipHostFree(ctx->h_den);
     if (ctx->h_req) (void)hipHostFree(ctx->h_req);
@@ -874,6 +880,140 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
         ctx->error = std::string("internal error: ") + e.what();
         (void)hipStreamSynchronize(ctx->compute);
         (void)hipStreamSynchronize(ctx->copy);
+        return RPS_EINTERNAL;
+    }
+}
+
+int rps_sieve_into(rps_ctx *ctx, const rps_request *req, int64_t *numerators,
+                   int32_t *denominators, uint64_t capacity, uint64_t *survivor_count) {
+    if (!ctx) {
+        return RPS_EINVAL;
+    }
+    ctx->error.clear();
+    ctx->last_kernel_ms = 0.0;
+    ctx->last_launches = 0;
+    ctx->last_sites = 0.0;
+    if (survivor_count) {
+        *survivor_count = 0;
+    }
+    try {
+        if (!numerators || !denominators || !survivor_count) {
+            throw ApiError{RPS_EINVAL, "null output pointer"};
+        }
+        validate(req);
+        RPS_HIP(hipSetDevice(ctx->device));
+        unsigned char *scratch = static_cast<unsigned char *>(rps::scratch_address(ctx->device));
+        if (!scratch) {
+            throw ApiError{RPS_EINTERNAL, "device scratch not initialised"};
+        }
+        namespace S = rps::scratch_layout;
+        auto *resid = reinterpret_cast<unsigned long long *>(scratch + S::kResid);
+        auto *chi = reinterpret_cast<unsigned long long *>(scratch + S::kChi);
+        auto *coef = reinterpret_cast<int *>(scratch + S::kCoef);
+        auto *negmod = reinterpret_cast<int *>(scratch + S::kNegMod);
+        auto *state = reinterpret_cast<rps::DeviceState *>(scratch + S::kState);
+        auto *cursors = reinterpret_cast<unsigned long long *>(scratch + S::kCursors);
+
+        const rps::LineDesc geometry = line_geometry(req);
+        const rps::VariantInfo *var = rps::variant(
+            ctx->variant >= 0 ? ctx->variant : tuned_variant(geometry.words));
+        const int stride = 64 * var->words_per_lane;
+        const int waves = std::max(var->waves, rps::kDirectWaves);
+        if (static_cast<unsigned long long>(waves) > S::kMaxWaves) {
+            throw ApiError{RPS_EINTERNAL, "variant has more waves than the scratch cursors hold"};
+        }
+        const unsigned long long words = geometry.words;
+        // Every launch must fit the scratch's cursors: a line alone is cut
+        // into at least ceil(words / 2^25) tiles.
+        if ((words + (1ULL << 25) - 1) >> 25 > S::kMaxTiles) {
+            throw ApiError{RPS_EINVAL, "maximum height exceeds the allocation-free limit "
+                                       "(lines of at most 2^39 words)"};
+        }
+
+        // One curve's tables into the scratch: coefficient residues and
+        // (-A) mod p staged in the context's pinned buffers.
+        const long long A = req->numerator_bound;
+        coefficient_residues(req, ctx->primes, req->n_coefficients, ctx->h_coef);
+        for (int i = 0; i < rps::kCandidates; ++i) {
+            const long long p = ctx->primes[i];
+            ctx->h_negmod[i] = static_cast<int>(((-A) % p + p) % p);
+        }
+        RPS_HIP(hipMemcpyAsync(coef, ctx->h_coef,
+                               static_cast<size_t>(rps::kCandidates) * req->n_coefficients
+                                   * sizeof(int),
+                               hipMemcpyHostToDevice, ctx->compute));
+        RPS_HIP(hipMemcpyAsync(negmod, ctx->h_negmod, rps::kCandidates * sizeof(int),
+                               hipMemcpyHostToDevice, ctx->compute));
+        RPS_HIP(rps::launch_chi(coef, req->n_coefficients, 1, ctx->d_primes, chi, ctx->compute));
+        RPS_HIP(rps::launch_resid(ctx->d_primes, chi, 1, nullptr, resid, ctx->compute));
+
+        // All batches reserve from one flush state, reset once: survivors go
+        // to [0, count) of the caller's buffers; a reservation that does not
+        // fit is recorded and every later one fails too, so the written
+        // survivors stay exactly [0, min(count, first_fail)).
+        rps::SieveParams base{};
+        base.line = geometry;
+        base.line.resid = resid;
+        base.line.neg_bound_mod = negmod;
+        base.primes = ctx->d_primes;
+        base.out.num = reinterpret_cast<long long *>(numerators);
+        base.out.den = denominators;
+        base.out.req = nullptr;
+        base.out.state = state;
+        base.out.capacity = capacity;
+        base.out.primitive = ctx->primitive;
+        base.cursors = cursors;
+        base.resume = 0;
+        RPS_HIP(rps::launch_reset(state, ctx->compute));
+        const double line_sites = 2.0 * static_cast<double>(A) + 1.0;
+        for (long long next_t = req->den_first; next_t <= req->den_last;) {
+            long long n = std::min<long long>(ctx->batch_size, req->den_last - next_t + 1);
+            n = std::min<long long>(n, static_cast<long long>(S::kMaxTiles));
+            rps::SieveParams prm = base;
+            if (words <= ctx->direct_words) {
+                direct_tiling(prm.line, std::max(1LL, n / std::max(1LL, ctx->min_tiles)));
+            } else {
+                unsigned long long chunks = line_chunks(ctx->min_tiles, n, words, stride,
+                                                        var->waves);
+                n = std::min<long long>(n, static_cast<long long>(S::kMaxTiles / chunks));
+                chunks = line_chunks(ctx->min_tiles, n, words, stride, var->waves);
+                row_tiling(prm.line, chunks, stride);
+                row_runs(prm.line, std::max(1LL, n / std::max(1LL, ctx->min_tiles)),
+                         var->run_log2);
+            }
+            prm.line.first_t = static_cast<int>(next_t);
+            prm.line.n_t = static_cast<int>(n);
+            const size_t tiles = tile_count(prm.line);
+            if (tiles > S::kMaxTiles) {
+                throw ApiError{RPS_EINTERNAL, "batch exceeds the scratch's tile limit"};
+            }
+            if (prm.line.direct) {
+                RPS_HIP(rps::launch_direct(prm, static_cast<unsigned>(tiles), ctx->compute));
+            } else {
+                RPS_HIP(var->launch(prm, static_cast<unsigned>(tiles), ctx->compute));
+            }
+            ++ctx->last_launches;
+            ctx->last_sites += line_sites * static_cast<double>(n);
+            next_t += n;
+        }
+        rps::DeviceState *h_state = ctx->slots[0].h_state;
+        RPS_HIP(hipMemcpyAsync(h_state, state, sizeof(rps::DeviceState), hipMemcpyDeviceToHost,
+                               ctx->compute));
+        RPS_HIP(hipStreamSynchronize(ctx->compute));
+        const unsigned long long valid = std::min(h_state->count, h_state->first_fail);
+        *survivor_count = std::min<unsigned long long>(valid, capacity);
+        if (h_state->full) {
+            throw ApiError{RPS_EOVERFLOW, "survivors exceed the output capacity of "
+                                          + std::to_string(capacity) + " entries"};
+        }
+        return RPS_OK;
+    } catch (const HipError &e) {
+        return fail(ctx, e);
+    } catch (const ApiError &e) {
+        return fail(ctx, e);
+    } catch (const std::exception &e) {
+        ctx->error = std::string("internal error: ") + e.what();
+        (void)hipStreamSynchronize(ctx->compute);
         return RPS_EINTERNAL;
     }
 }

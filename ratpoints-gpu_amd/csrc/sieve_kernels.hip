@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include <gfx950_asm.hpp>
 
@@ -1620,6 +1621,45 @@ hipError_t launch_direct(const SieveParams &params, unsigned grid,
     hipLaunchKernelGGL((direct_kernel<kDirectWaves, kDirectMaxDenominators>), dim3(grid),
                        dim3(kDirectWaves * 64), 0, stream, params);
     return hipGetLastError();
+}
+
+// The allocation-free entry point's partial results (residue table, chi
+// bitmaps, coefficient residues, flush state, cursors) live here, one copy
+// per device, for the life of the process.  Host code never passes the
+// array's name to a kernel: it holds no device address on the host side.
+// scratch_init stores the address the runtime reports for the current
+// device, and launches use that.
+static __device__ __align__(256) unsigned char scratch[scratch_layout::kBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_lock;
+void *g_scratch[kMaxScratchDevices] = {};
+}  // namespace
+
+hipError_t scratch_init(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) {
+        return hipErrorInvalidDevice;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_lock);
+    if (g_scratch[device]) {
+        return hipSuccess;
+    }
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        return e;
+    }
+    g_scratch[device] = ptr;
+    return hipSuccess;
+}
+
+void *scratch_address(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) {
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_lock);
+    return g_scratch[device];
 }
 
 hipError_t launch_reset(DeviceState *state, hipStream_t stream) {

@@ -47,6 +47,14 @@ constexpr unsigned kDirectMaxWords = 768;
 hipError_t launch_direct(const SieveParams &params, unsigned grid,
                          hipStream_t stream);
 
+// Device scratch of the allocation-free entry point: one file-scope array of
+// scratch_layout::kBytes per device (layout in sieve_layout.hpp).  scratch_init
+// resolves the array's address on `device`, which must be the current
+// device; call it once per device at set-up.  scratch_address returns that
+// address, or nullptr when scratch_init has not run for the device.
+hipError_t scratch_init(int device);
+void *scratch_address(int device);
+
 }  // namespace rps
 
 #endif

@@ -109,6 +109,25 @@ struct MultiParams {
     int bucket_shift;
 };
 
+// Layout of the allocation-free entry point's device scratch (byte offsets,
+// 256-aligned): one curve's residue table, chi bitmaps and coefficient
+// residues, (-A) mod p, the flush state, and the per-wave cursors of at most
+// kMaxTiles tiles of at most kMaxWaves waves.
+namespace scratch_layout {
+constexpr unsigned long long align256(unsigned long long n) { return (n + 255) / 256 * 256; }
+constexpr unsigned long long kMaxTiles = 16384;
+constexpr unsigned long long kMaxWaves = 16;
+constexpr unsigned long long kResid = 0;
+constexpr unsigned long long kChi = align256(kResid + kResidTableWords * 8);
+constexpr unsigned long long kCoef =
+    align256(kChi + static_cast<unsigned long long>(kCandidates) * kChiWords * 8);
+constexpr unsigned long long kNegMod =
+    align256(kCoef + static_cast<unsigned long long>(kCandidates) * kMaxCoefficients * 4);
+constexpr unsigned long long kState = align256(kNegMod + kCandidates * 4);
+constexpr unsigned long long kCursors = align256(kState + sizeof(DeviceState));
+constexpr unsigned long long kBytes = align256(kCursors + kMaxTiles * kMaxWaves * 8);
+}  // namespace scratch_layout
+
 }  // namespace rps
 
 #endif
