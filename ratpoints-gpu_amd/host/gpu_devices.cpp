@@ -1,6 +1,6 @@
 // Device discovery over the C ABI (rps_device_count / rps_device_info).
-// Selection syntax and messages follow src/gpu_devices.cpp:21-97 of the
-// reference; a search uses exactly one device.
+// The --devices syntax and its messages are the reference's interface
+// (src/gpu_devices.cpp:21-53); a search uses exactly one device.
 #include "gpu_devices.hpp"
 
 #include <algorithm>
@@ -16,38 +16,61 @@ thread_local int bound_device = 0;
 
 }  // namespace
 
+namespace {
+
+// The comma-separated fields of a device list ("" gives one empty field).
+std::vector<std::string> comma_fields(const std::string &text) {
+    std::vector<std::string> fields(1);
+    for (char c : text) {
+        if (c == ',') {
+            fields.emplace_back();
+        } else {
+            fields.back() += c;
+        }
+    }
+    return fields;
+}
+
+// A device id: decimal digits only, within int.
+bool device_id(const std::string &field, int &id) {
+    long long v = 0;
+    for (char c : field) {
+        if (c < '0' || c > '9') {
+            return false;
+        }
+        v = 10 * v + (c - '0');
+        if (v > std::numeric_limits<int>::max()) {
+            return false;
+        }
+    }
+    id = static_cast<int>(v);
+    return true;
+}
+
+}  // namespace
+
 std::vector<int> parse_device_selection(const std::string &text) {
     if (text == "all") {
         return {};
     }
+    const std::vector<std::string> fields = comma_fields(text);
     std::vector<int> ids;
-    size_t at = 0;
-    while (at < text.size()) {
-        size_t comma = text.find(',', at);
-        if (comma == std::string::npos) {
-            comma = text.size();
-        }
-        if (comma == at) {
-            throw std::invalid_argument("empty GPU ID in --devices");
+    for (size_t k = 0; k < fields.size(); ++k) {
+        if (fields[k].empty()) {
+            // A trailing empty field means a trailing comma (or no list).
+            throw std::invalid_argument(k + 1 == fields.size()
+                ? "--devices requires all or a nonempty GPU list"
+                : "empty GPU ID in --devices");
         }
         int id = 0;
-        for (size_t i = at; i < comma; ++i) {
-            const int digit = text[i] - '0';
-            if (digit < 0 || digit > 9
-                || id > (std::numeric_limits<int>::max() - digit) / 10) {
-                throw std::invalid_argument(
-                    "--devices requires all or comma-separated nonnegative GPU IDs");
-            }
-            id = 10 * id + digit;
+        if (!device_id(fields[k], id)) {
+            throw std::invalid_argument(
+                "--devices requires all or comma-separated nonnegative GPU IDs");
         }
-        if (std::find(ids.begin(), ids.end(), id) != ids.end()) {
+        if (std::count(ids.begin(), ids.end(), id)) {
             throw std::invalid_argument("duplicate GPU ID in --devices");
         }
         ids.push_back(id);
-        at = comma + 1;
-    }
-    if (ids.empty() || text.back() == ',') {
-        throw std::invalid_argument("--devices requires all or a nonempty GPU list");
     }
     return ids;
 }

@@ -4,6 +4,7 @@
 #define RATPOINTS_GPU_OUTPUT_FORMATTER_HPP
 
 #include <string>
+#include <vector>
 
 #include "search_types.hpp"
 
@@ -32,18 +33,21 @@ public:
     void finish();
 
 private:
-    static std::string unescape(const std::string &text);
-    static void emit(const std::string &text);
-    void write_point(long long numerator, const std::string &ordinate,
-                     int denominator);
+    // One piece of a compiled point format: literal text or a field.
+    struct Piece {
+        enum Kind { kText, kNumerator, kOrdinate, kDenominator } kind;
+        std::string text;
+    };
+
+    void put_point(long long numerator, const std::string &ordinate, int denominator);
 
     bool print_points_;
     bool print_y_;
-    std::string format_;
+    std::vector<Piece> format_;
     std::string before_;
     std::string between_;
     std::string after_;
-    bool wrote_point_ = false;
+    bool any_point_ = false;
     bool finished_ = false;
 };
 

@@ -300,3 +300,22 @@ def test_record_curve_stdout_identical_to_reference(name):
     assert hashlib.sha256(proc.stdout).hexdigest() == row["output_sha256"]
     m = re.search(rb"survivors=(\d+) exact_survivors=(\d+)", proc.stderr)
     assert (int(m[1]), int(m[2])) == (row["survivors"], row["exact"])
+
+
+CLI_BEHAVIOUR = __import__("json").load(
+    open(os.path.join(ROOT, "tests", "golden", "cli_behaviour.json")))
+CHUNK = 40
+
+
+@pytest.mark.parametrize("binary", BINARIES)
+@pytest.mark.parametrize("start", range(0, len(CLI_BEHAVIOUR), CHUNK))
+def test_cli_behaviour_fixture(binary, start):
+    # Exit status, stdout and stderr byte for byte on argument lists probing
+    # every parsing rule, message and output escape of the command line
+    # (tests/golden/make_cli_fixture.py).
+    env = dict(os.environ)
+    env.pop("RATPOINTS_GPU_BENCHMARK", None)
+    for row in CLI_BEHAVIOUR[start:start + CHUNK]:
+        proc = subprocess.run([binary, *row["args"]], capture_output=True, timeout=60, env=env)
+        got = (proc.returncode, proc.stdout.decode("latin-1"), proc.stderr.decode("latin-1"))
+        assert got == (row["rc"], row["stdout"], row["stderr"]), row["args"]
