@@ -70,11 +70,12 @@ def build_library():
     return out
 
 
-def build_cli(backend=None, out=None, sources=None, name="ratpoints_gpu"):
+def build_cli(backend=None, out=None, sources=None, name="ratpoints_gpu", extra=()):
     """The CLI over the HIP library, or -- for tests only -- over another
-    implementation of the C ABI given as a list of object files."""
+    implementation of the C ABI given as a list of object files; `extra` adds
+    source files by path."""
     host = os.path.join(HERE, "host")
-    sources = [os.path.join(host, s) for s in (sources or HOST_SOURCES)]
+    sources = [os.path.join(host, s) for s in (sources or HOST_SOURCES)] + list(extra)
     if not all(os.path.exists(s) for s in sources):
         return None
     prefix = gmp_prefix()
@@ -85,7 +86,9 @@ def build_cli(backend=None, out=None, sources=None, name="ratpoints_gpu"):
     libdir = os.path.join(HERE, "lib")
     cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-Wextra", "-pthread",
            "-I" + host, "-I" + os.path.join(ROOT, "include")]
-    rpath = "-Wl,-rpath,$ORIGIN/../lib"
+    # The library directory relative to the binary, so the tree can move.
+    origin_lib = "$ORIGIN/" + os.path.relpath(libdir, os.path.dirname(os.path.abspath(out)))
+    rpath = "-Wl,-rpath," + origin_lib
     if prefix != "/usr":
         # GMP is linked by full path and found at run time through a
         # directory of symlinks to it alone, so the prefix's other libraries
@@ -98,7 +101,7 @@ def build_cli(backend=None, out=None, sources=None, name="ratpoints_gpu"):
                 os.remove(link)
             os.symlink(os.path.join(prefix, "lib", soname), link)
         cmd += ["-I" + os.path.join(prefix, "include")]
-        rpath += ",-rpath,$ORIGIN/../lib/gmp"
+        rpath += ",-rpath," + origin_lib + "/gmp"
         gmp_libs = [os.path.join(prefix, "lib", "libgmpxx.so"),
                     os.path.join(prefix, "lib", "libgmp.so")]
     else:
@@ -107,7 +110,7 @@ def build_cli(backend=None, out=None, sources=None, name="ratpoints_gpu"):
         abi = ["-L" + libdir, "-lratpoints_sieve"]
     else:
         abi = list(backend)
-        rpath = rpath.replace("$ORIGIN/../lib", libdir)
+        rpath = rpath.replace(origin_lib, libdir)
     cmd += [*sources, "-o", out + ".tmp", *abi, rpath, *gmp_libs]
     run(cmd)
     os.replace(out + ".tmp", out)
@@ -140,6 +143,45 @@ def build_oracle():
     build_serve(backend=objs, out=os.path.join(build_dir, "ratpoints_serve_oracle"))
 
 
+HOST_CLANG = os.environ.get("HOST_CLANG", "/opt/rocm/llvm/bin/clang++")
+
+
+def build_emulator():
+    """Test infrastructure: the library's own sources (kernels and C ABI)
+    compiled for the host against the CPU wave emulator
+    (tests/emu/_build/libratpoints_sieve_emu.so), which the CPU tests drive
+    through the same C ABI.  Never loaded by the package."""
+    emu = os.path.join(ROOT, "tests", "emu")
+    out_dir = os.path.join(emu, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "libratpoints_sieve_emu.so")
+    run([HOST_CLANG, "-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall",
+         "-Wno-unknown-attributes", "-Wno-unused-function",
+         "-I" + os.path.join(emu, "include"), "-I" + os.path.join(ROOT, "include"),
+         "-I" + os.path.join(HERE, "csrc"),
+         "-x", "c++", os.path.join(HERE, "csrc", "sieve_kernels.hip"),
+         "-x", "none", os.path.join(HERE, "csrc", "rps_abi.cpp"),
+         os.path.join(emu, "wave_emu.cpp"), "-shared", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_test_programs():
+    """Test infrastructure: the C++ streaming-contract test over the HIP
+    library (tests/cpp/_build/streaming_contract) and over the emulator
+    (streaming_contract_emu), and the serving driver over the emulator."""
+    emu_lib = build_emulator()
+    cpp = os.path.join(ROOT, "tests", "cpp")
+    out_dir = os.path.join(cpp, "_build")
+    contract = [os.path.join(cpp, "streaming_contract.cpp")]
+    shim = ["exact_polynomial.cpp", "sieve_shim.cpp", "gpu_devices.cpp"]
+    build_cli(sources=shim, extra=contract, out=os.path.join(out_dir, "streaming_contract"))
+    build_cli(backend=[emu_lib], sources=shim, extra=contract,
+              out=os.path.join(out_dir, "streaming_contract_emu"))
+    build_serve(backend=[emu_lib],
+                out=os.path.join(ROOT, "tests", "emu", "_build", "ratpoints_serve_emu"))
+
+
 def build_all():
     if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
@@ -147,6 +189,7 @@ def build_all():
     cli = build_cli()
     build_serve()
     build_oracle()
+    build_test_programs()
     return lib, cli
 
 

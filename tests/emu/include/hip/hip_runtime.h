@@ -6,7 +6,9 @@
 // lanes of the wave have arrived at the same operation from the same source
 // line; a wave whose lanes arrive at different operations is reported as
 // divergent.  __syncthreads releases once every wave that has not finished
-// waits at it.  Workgroups of a launch run one after another, so __shared__
+// waits at it.  Waves run one after another between barriers, or
+// (RPS_EMU_SCHEDULE=interleave|random) one cross-lane operation at a time in
+// turn, in a fixed or a seeded random order.  Workgroups of a launch run one after another, so __shared__
 // variables become function statics shared by the lanes of the running
 // workgroup.  A stuck launch (a lane looping without reaching an operation,
 // or a barrier no other wave reaches) is reported with the position of every
@@ -26,7 +28,10 @@
 #define __host__
 #define __forceinline__ inline __attribute__((always_inline))
 #define __launch_bounds__(...)
-#define __shared__ static
+// All __shared__ variables share one section, which the emulator can fill
+// with garbage before each workgroup (RPS_EMU_POISON_LDS=1) so that a read
+// of LDS no lane of the workgroup wrote shows up.
+#define __shared__ static __attribute__((section("rps_emu_lds")))
 #define __align__(n) __attribute__((aligned(n)))
 
 struct dim3 {

@@ -16,6 +16,7 @@
 #include <sys/time.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <csignal>
@@ -328,7 +329,103 @@ void resolve(int w, int op) {
     }
 }
 
+// Schedule and LDS poisoning, from the environment.
+struct Options {
+    int schedule = 0;  // 0 sequential, 1 interleave, 2 random
+    bool poison = false;
+    uint64_t seed = 1;
+};
+
+const Options &options() {
+    static const Options o = [] {
+        Options r;
+        const char *s = std::getenv("RPS_EMU_SCHEDULE");
+        if (s && std::string(s) == "interleave") {
+            r.schedule = 1;
+        } else if (s && std::string(s) == "random") {
+            r.schedule = 2;
+        }
+        const char *p = std::getenv("RPS_EMU_POISON_LDS");
+        r.poison = p && std::string(p) == "1";
+        const char *seed = std::getenv("RPS_EMU_SEED");
+        r.seed = seed ? std::strtoull(seed, nullptr, 10) : 1;
+        return r;
+    }();
+    return o;
+}
+
+uint64_t g_rng = 0;
+
+uint64_t next_random() {
+    g_rng += 0x9E3779B97F4A7C15ULL;
+    uint64_t z = g_rng;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+}  // namespace emu
+
+extern "C" char __start_rps_emu_lds[] __attribute__((weak));
+extern "C" char __stop_rps_emu_lds[] __attribute__((weak));
+
+namespace emu {
+namespace {
+
+void poison_lds() {
+    if (!__start_rps_emu_lds || !__stop_rps_emu_lds) {
+        return;
+    }
+    const int byte = static_cast<int>(next_random() & 0xFF);
+    std::memset(__start_rps_emu_lds, byte, static_cast<size_t>(__stop_rps_emu_lds - __start_rps_emu_lds));
+}
+
+// Runs the runnable lanes of wave w to their next operation and settles
+// the wave: 0 resolved an operation, 1 all lanes wait at a barrier, 2 all
+// lanes have returned.
+int step_wave(int w) {
+    const int base = w * kWaveSize;
+    const int n = std::min(kWaveSize, g.n_lanes - base);
+    for (int l = 0; l < n; ++l) {
+        if (g.lanes[base + l].op == kRunnable) {
+            run_lane(g.lanes[base + l]);
+        }
+    }
+    int live = 0, op = -1, line = 0;
+    bool mixed = false;
+    for (int l = 0; l < n; ++l) {
+        const Lane &x = g.lanes[base + l];
+        if (x.op == kExited) {
+            continue;
+        }
+        if (live++ == 0) {
+            op = x.op;
+            line = x.line;
+        } else if (x.op != op || x.line != line) {
+            mixed = true;
+        }
+    }
+    if (live == 0) {
+        return 2;
+    }
+    if (mixed) {
+        fatal("lanes of one wave wait at different operations (divergent "
+              "cross-lane operation or barrier)");
+    }
+    if (op == kBarrier) {
+        return 1;
+    }
+    resolve(w, op);
+    return 0;
+}
+
+bool g_poison_launch = false;  // poison before each workgroup of this launch
+
 void run_block() {
+    if (g_poison_launch) {
+        poison_lds();
+    }
     std::vector<char> done(g.n_waves, 0), parked(g.n_waves, 0);
     for (int i = 0; i < g.n_lanes; ++i) {
         Lane &l = g.lanes[i];
@@ -342,46 +439,39 @@ void run_block() {
         l.op = kRunnable;
         l.line = 0;
     }
+    std::vector<int> order(g.n_waves);
+    for (int w = 0; w < g.n_waves; ++w) {
+        order[w] = w;
+    }
     for (;;) {
-        for (int w = 0; w < g.n_waves; ++w) {
-            if (done[w] || parked[w]) {
-                continue;
+        if (options().schedule == 0) {
+            // Each wave runs until it waits at a barrier or returns.
+            for (int w = 0; w < g.n_waves; ++w) {
+                while (!done[w] && !parked[w]) {
+                    const int r = step_wave(w);
+                    done[w] = r == 2;
+                    parked[w] = r == 1;
+                }
             }
-            const int base = w * kWaveSize;
-            const int n = std::min(kWaveSize, g.n_lanes - base);
-            for (;;) {
-                for (int l = 0; l < n; ++l) {
-                    if (g.lanes[base + l].op == kRunnable) {
-                        run_lane(g.lanes[base + l]);
+        } else {
+            // One operation per wave in turn (fixed or shuffled order),
+            // until every wave waits at a barrier or has returned.
+            for (bool moved = true; moved;) {
+                moved = false;
+                if (options().schedule == 2) {
+                    for (int k = g.n_waves - 1; k > 0; --k) {
+                        std::swap(order[k], order[next_random() % (k + 1)]);
                     }
                 }
-                int live = 0, op = -1, line = 0;
-                bool mixed = false;
-                for (int l = 0; l < n; ++l) {
-                    const Lane &x = g.lanes[base + l];
-                    if (x.op == kExited) {
+                for (int w : order) {
+                    if (done[w] || parked[w]) {
                         continue;
                     }
-                    if (live++ == 0) {
-                        op = x.op;
-                        line = x.line;
-                    } else if (x.op != op || x.line != line) {
-                        mixed = true;
-                    }
+                    const int r = step_wave(w);
+                    done[w] = r == 2;
+                    parked[w] = r == 1;
+                    moved = true;
                 }
-                if (live == 0) {
-                    done[w] = 1;
-                    break;
-                }
-                if (mixed) {
-                    fatal("lanes of one wave wait at different operations (divergent "
-                          "cross-lane operation or barrier)");
-                }
-                if (op == kBarrier) {
-                    parked[w] = 1;
-                    break;
-                }
-                resolve(w, op);
             }
         }
         bool all_done = true;
@@ -464,6 +554,14 @@ void launch(const char *name, dim3 grid, dim3 block, Entry entry, void *closure)
     g.n_lanes = static_cast<int>(threads);
     g.n_waves = (g.n_lanes + kWaveSize - 1) / kWaveSize;
     g.active = true;
+    if (g_rng == 0) {
+        g_rng = options().seed;
+    }
+    // The per-curve table kernels write their LDS before reading it and run
+    // thousands of workgroups: only the sieve kernels' LDS is poisoned.
+    const std::string kernel(name);
+    g_poison_launch = options().poison && kernel.find("resid_kernel") == std::string::npos
+        && kernel.find("chi_kernel") == std::string::npos;
     arm_watchdog(true);
     for (unsigned z = 0; z < grid.z; ++z) {
         for (unsigned y = 0; y < grid.y; ++y) {

@@ -1,0 +1,156 @@
+"""The HIP library's own sources built for the CPU wave emulator
+(tests/emu/_build/libratpoints_sieve_emu.so), driven through the same C ABI
+as the GPU tests.
+
+Bar: survivor multisets bit-identical to the CPU oracle's (or to rps_sieve's
+for the other entry points), with no barrier, cross-lane or watchdog report
+from the emulator.  The emulator runs every lane of a workgroup as a fiber,
+checks that the lanes of a wave meet at the same cross-lane operation and
+that the waves meet at the same barrier, and here also fills the kernels'
+LDS with garbage before each workgroup and interleaves the waves in a seeded
+random order, so a read of LDS no lane wrote or a race between waves shows
+up as a wrong survivor set.  These tests check the kernels' control flow and
+indexing on the CPU; the GPU tests check the same cases on the device.
+"""
+
+import os
+import random
+import subprocess
+
+import numpy as np
+import pytest
+
+import curves
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EMU_DIR = os.path.join(ROOT, "tests", "emu", "_build")
+EMU_LIB = os.path.join(EMU_DIR, "libratpoints_sieve_emu.so")
+EMU_SERVE = os.path.join(EMU_DIR, "ratpoints_serve_emu")
+CONTRACT = os.path.join(ROOT, "tests", "cpp", "_build", "streaming_contract_emu")
+
+SCHEDULE = {"RPS_EMU_SCHEDULE": "random", "RPS_EMU_POISON_LDS": "1", "RPS_EMU_SEED": "11",
+            "RPS_EMU_WATCHDOG_S": "300"}
+
+
+@pytest.fixture(scope="module")
+def emu(pkg):
+    if not os.path.exists(EMU_LIB):
+        pytest.skip("emulated library not built")
+    os.environ.update(SCHEDULE)  # read once by the emulator, at its first launch
+    lib = pkg.bind_library(EMU_LIB)
+    ctx = pkg.SieveContext(0, lib=lib)
+    yield ctx
+    ctx.close()
+
+
+def defaults(ctx):
+    for name, value in (("variant", -1), ("survivor_capacity", 1 << 24), ("batch_size", 65536),
+                        ("min_tiles", 2048), ("direct_words", 768), ("primitive_only", 0)):
+        ctx.set_option(name, value)
+    return ctx
+
+
+def same(res, ref):
+    count, digest, num, den = ref
+    return (res.survivor_count == count and np.array_equal(res.numerators, num)
+            and np.array_equal(res.denominators, den))
+
+
+def test_every_variant_matches_oracle(emu, pkg, oracle_lib):
+    # One long-line case per compiled variant: lines of 1,251 words in runs
+    # (min_tiles 1), denominators divisible by candidate primes included.
+    coeffs, _ = curves.partially_dense(random.Random(5), 7, 6)
+    ref = oracle_lib.sieve(coeffs, 40000, 437, 446, method="words")
+    for v, name in enumerate(pkg.variants()):
+        defaults(emu).set_option("variant", v)
+        emu.set_option("min_tiles", 1)
+        assert same(emu.sieve(coeffs, 40000, 437, 446), ref), name
+
+
+def test_resume_exactly_once(emu, pkg, oracle_lib):
+    # About a quarter of all sites survive: a 16,384-entry buffer overflows
+    # in every launch and the waves resume at their cursors.
+    coeffs, _ = curves.partially_dense(random.Random(78), 2, 4)
+    ref = oracle_lib.sieve(coeffs, 20000, 3, 6, method="words")
+    for name in ("q_u1_f12_w8", "w_u2_f14_w16"):
+        defaults(emu).set_option("variant", pkg.variants().index(name))
+        emu.set_option("survivor_capacity", 16384)
+        res = emu.sieve(coeffs, 20000, 3, 6)
+        assert res.stats["launches"] > 2
+        assert same(res, ref), name
+
+
+def test_short_lines_and_square_mode(emu, oracle_lib):
+    coeffs, _ = curves.partially_dense(random.Random(9), 6, 5)
+    defaults(emu)
+    assert same(emu.sieve(coeffs, 3000, 1, 40), oracle_lib.sieve(coeffs, 3000, 1, 40,
+                                                                  method="words"))
+    assert same(emu.sieve(coeffs, 9000, 46300, 46340, square=True),
+                oracle_lib.sieve(coeffs, 9000, 46300, 46340, square=True, method="words"))
+
+
+def test_many_curves_mixed_tiles(emu, oracle_lib):
+    # Row runs, chunked lines, wide-row lines, short lines and square
+    # denominators in one call: two persistent launches.
+    rng = random.Random(21)
+    reqs = []
+    for bound, first, last, square in ((90000, 1, 3, False), (3000, 5, 60, False),
+                                       (40, 1, 200, False), (12000, 7, 9, False),
+                                       (6000, 45000, 45010, True), (1, 1, 30, False)):
+        coeffs, _ = curves.partially_dense(rng, rng.randint(6, 10), rng.choice([3, 4, 6]))
+        reqs.append({"coefficients": coeffs, "bound": bound, "first": first, "last": last,
+                     "square": square})
+    results, stats = defaults(emu).sieve_many(reqs)
+    assert stats["requeued"] == 0
+    for r, (num, den) in zip(reqs, results):
+        count, digest, on, od = oracle_lib.sieve(r["coefficients"], r["bound"], r["first"],
+                                                 r["last"], r["square"], method="words")
+        assert np.array_equal(num, on) and np.array_equal(den, od), r["bound"]
+
+
+def test_sieve_into_matches_sieve(emu):
+    coeffs = ["1", "-3", "0", "2", "0", "0", "4"]
+    defaults(emu)
+    for bound, first, last, cap in ((300000, 1, 20, 4096), (3000, 1, 30, 4096)):
+        ref = emu.sieve(coeffs, bound, first, last)
+        num = np.zeros(cap, np.int64)
+        den = np.zeros(cap, np.int32)
+        rc, n = emu.sieve_into(coeffs, bound, first, last, num.ctypes.data, den.ctypes.data, cap)
+        order = np.lexsort((num[:n], den[:n]))
+        assert rc == 0 and n == ref.survivor_count
+        assert np.array_equal(num[:n][order], ref.numerators)
+        assert np.array_equal(den[:n][order], ref.denominators)
+
+
+def test_streaming_contract_through_reference_signature():
+    if not os.path.exists(CONTRACT):
+        pytest.skip("streaming_contract_emu not built")
+    env = dict(os.environ, **SCHEDULE)
+    for bound, last, sites in (("5000", "2", 20002), ("30000", "3", 180003)):
+        proc = subprocess.run([CONTRACT, bound, last], capture_output=True, text=True,
+                              timeout=600, env=env)
+        assert proc.returncode == 0, proc.stderr
+        ok, got, flushes = proc.stdout.split()
+        assert ok == "ok" and int(got) == sites and int(flushes) >= 2
+
+
+def test_mode_coverage_trace_through_serving_loop(workloads, tmp_path):
+    # Square-denominator cubics and quartics and degree 50-100 curves with
+    # 1,024-bit coefficients, intervals and infinity, all at once through
+    # the serving loop's many-curve rounds, then one at a time.
+    if not os.path.exists(EMU_SERVE):
+        pytest.skip("ratpoints_serve_emu not built")
+    import test_serve
+    reqs = test_serve.mode_trace(workloads, 8, square_k=40, square_bound=60, high_height=12)
+    trace = tmp_path / "trace.txt"
+    workloads.write_trace(str(trace), reqs)
+    env = dict(os.environ, **SCHEDULE)
+    results = {}
+    for mode in ("serve", "alone"):
+        out = tmp_path / f"{mode}.txt"
+        proc = subprocess.run([EMU_SERVE, "--trace", str(trace), "--mode", mode, "--out",
+                               str(out), "--round-sites", "3000", "--max-active", "8"],
+                              capture_output=True, text=True, timeout=900, env=env)
+        assert proc.returncode == 0, proc.stderr
+        results[mode] = test_serve.parse_results(out)
+    test_serve.check(reqs, results, oracle_limit=10 ** 9)

@@ -406,3 +406,19 @@ def test_short_lines_square_mode(fresh_gpu, oracle_lib):
     announce(case="short-square")
     res = fresh_gpu.sieve(coeffs, 700, 46000, 46340, square=True)
     assert_same(res, oracle_lib.sieve(coeffs, 700, 46000, 46340, square=True, method="words"))
+
+
+def test_streaming_contract_through_reference_signature():
+    # ratpoints_gpu::run_modular_sieve called from C++ with the reference's
+    # signature (tests/test_sieve_streaming.cpp:45-80 in the reference): every
+    # site of 1 + P x delivered exactly once, over a 16,384-entry buffer so
+    # the stream comes in several flushes.
+    binary = os.path.join(os.path.dirname(__file__), "cpp", "_build", "streaming_contract")
+    assert os.path.exists(binary), "tests/cpp/_build/streaming_contract not built"
+    for bound, last, sites in ((5000, 2, 20002), (60000, 8, 960008)):
+        announce(case="streaming-contract", A=bound, B=last)
+        proc = __import__("subprocess").run([binary, str(bound), str(last)], capture_output=True,
+                                             text=True, timeout=120)
+        assert proc.returncode == 0, proc.stderr
+        ok, got, flushes = proc.stdout.split()
+        assert ok == "ok" and int(got) == sites and int(flushes) >= 2
