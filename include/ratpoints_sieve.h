@@ -193,6 +193,11 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests,
  *                        before they are copied (they can never be points;
  *                        the exact stage, src/point_search.cpp:48-60, skips
  *                        them).  Default 0: the raw modular survivor set.
+ *   "launch_timeout_ms"  longest wait for one kernel launch, 1..2^40 ms
+ *                        (default 600,000): a launch still running then
+ *                        fails the call with RPS_EDEVICE naming it, and the
+ *                        context refuses further work (close it; its device
+ *                        memory stays allocated until the process ends)
  *   "table_slots"        residue tables (1.33 MB each) kept on the device
  *                        between rps_sieve_many calls, least recently used
  *                        replaced, 1..65536 (default 256); a call with more

@@ -19,6 +19,7 @@
 #include <map>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ratpoints_sieve.h"
@@ -114,6 +115,33 @@ std::string op_name(const char *expr) {
     return paren == std::string::npos ? s : s.substr(0, paren);
 }
 
+// Polls `event` until it completes or `timeout_ms` passes; on expiry throws
+// RPS_EDEVICE naming the wait.  Used for every wait on a sieve kernel, so a
+// kernel that does not finish ends the call with an error instead of
+// blocking the caller.
+struct TimedOut {
+    std::string message;
+};
+
+void wait_event(hipEvent_t event, long long timeout_ms, const std::string &what) {
+    const auto deadline = Clock::now() + std::chrono::milliseconds(timeout_ms);
+    for (unsigned spins = 0;; ++spins) {
+        const hipError_t e = hipEventQuery(event);
+        if (e == hipSuccess) {
+            return;
+        }
+        if (e != hipErrorNotReady) {
+            throw HipError{e, "hipEventQuery"};
+        }
+        if (Clock::now() > deadline) {
+            throw TimedOut{what + " did not finish within " + std::to_string(timeout_ms) + " ms"};
+        }
+        if (spins > 64) {
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+        }
+    }
+}
+
 struct Slot {
     rps::DeviceState *d_state = nullptr;
     rps::DeviceState *h_state = nullptr;  // pinned
@@ -174,6 +202,7 @@ struct rps_ctx {
     int *d_negmod = nullptr;
     int *h_negmod = nullptr;  // pinned
     int *h_coef = nullptr;    // pinned [36][101]: coefficient residues for rps_sieve_into
+    unsigned long long *h_head = nullptr;  // pinned: tiles handed out by a many-curve launch
     std::vector<std::string> curve_key;
     bool curve_ready = false;
     Slot slots[2];
@@ -188,6 +217,11 @@ struct rps_ctx {
     long long min_tiles = 2048;
     unsigned long long direct_words = rps::kDirectMaxWords;
     int primitive = 0;  // emit only survivors with gcd(a, b) = 1
+    // A kernel wait longer than this fails the call (RPS_EDEVICE) instead
+    // of blocking; the context then refuses further work, since its streams
+    // still hold the kernel.
+    long long launch_timeout_ms = 600000;
+    bool broken = false;
     hipEvent_t basis_start = nullptr, basis_end = nullptr;
     double last_kernel_ms = 0.0;
     uint64_t last_launches = 0;
@@ -363,7 +397,7 @@ void prepare_curve(rps_ctx *ctx, const rps_request *req, rps_stats *st) {
     RPS_HIP(rps::launch_resid(ctx->d_primes, ctx->d_chi, 1, nullptr, ctx->d_resid,
                               ctx->compute));
     RPS_HIP(hipEventRecord(ctx->basis_end, ctx->compute));
-    RPS_HIP(hipEventSynchronize(ctx->basis_end));
+    wait_event(ctx->basis_end, ctx->launch_timeout_ms, "residue-table build");
     float ms = 0.0f;
     RPS_HIP(hipEventElapsedTime(&ms, ctx->basis_start, ctx->basis_end));
     if (st) {
@@ -447,6 +481,21 @@ int fail(rps_ctx *ctx, const ApiError &e) {
     return e.code;
 }
 
+// A kernel that did not finish: no stream is synchronised (that would block
+// behind it), and the context takes no more work.
+int fail(rps_ctx *ctx, const TimedOut &e) {
+    ctx->error = e.message + "; the context is unusable, close it";
+    ctx->broken = true;
+    ctx->curve_ready = false;
+    return RPS_EDEVICE;
+}
+
+void check_usable(const rps_ctx *ctx) {
+    if (ctx->broken) {
+        throw ApiError{RPS_EDEVICE, "context unusable after a kernel timeout; close it"};
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -490,6 +539,7 @@ int rps_open(int device, rps_ctx **out) {
         RPS_HIP(hipHostMalloc(&ctx->h_negmod, rps::kCandidates * sizeof(int)));
         RPS_HIP(hipHostMalloc(&ctx->h_coef, static_cast<size_t>(rps::kCandidates)
                                   * rps::kMaxCoefficients * sizeof(int)));
+        RPS_HIP(hipHostMalloc(&ctx->h_head, sizeof(unsigned long long)));
         RPS_HIP(hipEventCreate(&ctx->basis_start));
         RPS_HIP(hipEventCreate(&ctx->basis_end));
         for (Slot &s : ctx->slots) {
@@ -519,6 +569,12 @@ void rps_close(rps_ctx *ctx) {
     if (!ctx) {
         return;
     }
+    if (ctx->broken) {
+        // A kernel that timed out may still run: its streams and buffers
+        // are left to the process's end rather than waited for.
+        delete ctx;
+        return;
+    }
     (void)hipSetDevice(ctx->device);
     if (ctx->compute) (void)hipStreamSynchronize(ctx->compute);
     if (ctx->copy) (void)hipStreamSynchronize(ctx->copy);
@@ -532,6 +588,7 @@ void rps_close(rps_ctx *ctx) {
     if (ctx->d_negmod) (void)hipFree(ctx->d_negmod);
     if (ctx->h_negmod) (void)hipHostFree(ctx->h_negmod);
     if (ctx->h_coef) (void)hipHostFree(ctx->h_coef);
+    if (ctx->h_head) (void)hipHostFree(ctx->h_head);
     if (ctx->h_num) (void)hipHostFree(ctx->h_num);
     if (ctx->h_den) (void)hipHostFree(ctx->h_den);
     if (ctx->h_req) (void)hipHostFree(ctx->h_req);
@@ -598,6 +655,12 @@ int rps_set_option(rps_ctx *ctx, const char *name, int64_t value) {
             return RPS_EINVAL;
         }
         ctx->table_slots = value;
+    } else if (key == "launch_timeout_ms") {
+        if (value < 1 || value > (int64_t(1) << 40)) {
+            ctx->error = "launch_timeout_ms must be in 1..2^40";
+            return RPS_EINVAL;
+        }
+        ctx->launch_timeout_ms = value;
     } else if (key == "min_tiles") {
         if (value < 1 || value > (1 << 22)) {
             ctx->error = "min_tiles must be in 1..4194304";
@@ -690,6 +753,7 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
     ctx->last_sites = 0.0;
     const auto t_plan = Clock::now();
     try {
+        check_usable(ctx);
         if (!sink) {
             throw ApiError{RPS_EINVAL, "null sink"};
         }
@@ -790,7 +854,10 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
         // whether the batch still needs a resume launch.
         auto drain = [&](int si) -> bool {
             Slot &s = ctx->slots[si];
-            RPS_HIP(hipEventSynchronize(s.ready));
+            wait_event(s.ready, ctx->launch_timeout_ms,
+                       "sieve launch for denominators " + std::to_string(inflight[si].line.first_t)
+                           + ".." + std::to_string(inflight[si].line.first_t
+                                                   + inflight[si].line.n_t - 1));
             float kms = 0.0f;
             RPS_HIP(hipEventElapsedTime(&kms, s.k_start, s.k_end));
             st.sieve_ms += kms;
@@ -870,6 +937,8 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
         return cancelled ? RPS_ECANCELLED : RPS_OK;
     } catch (const HipError &e) {
         return fail(ctx, e);
+    } catch (const TimedOut &e) {
+        return fail(ctx, e);
     } catch (const ApiError &e) {
         return fail(ctx, e);
     } catch (const std::bad_alloc &) {
@@ -897,6 +966,7 @@ int rps_sieve_into(rps_ctx *ctx, const rps_request *req, int64_t *numerators,
         *survivor_count = 0;
     }
     try {
+        check_usable(ctx);
         if (!numerators || !denominators || !survivor_count) {
             throw ApiError{RPS_EINVAL, "null output pointer"};
         }
@@ -999,7 +1069,9 @@ int rps_sieve_into(rps_ctx *ctx, const rps_request *req, int64_t *numerators,
         rps::DeviceState *h_state = ctx->slots[0].h_state;
         RPS_HIP(hipMemcpyAsync(h_state, state, sizeof(rps::DeviceState), hipMemcpyDeviceToHost,
                                ctx->compute));
-        RPS_HIP(hipStreamSynchronize(ctx->compute));
+        RPS_HIP(hipEventRecord(ctx->slots[0].ready, ctx->compute));
+        wait_event(ctx->slots[0].ready, ctx->launch_timeout_ms,
+                   "rps_sieve_into's " + std::to_string(ctx->last_launches) + " launches");
         const unsigned long long valid = std::min(h_state->count, h_state->first_fail);
         *survivor_count = std::min<unsigned long long>(valid, capacity);
         if (h_state->full) {
@@ -1008,6 +1080,8 @@ int rps_sieve_into(rps_ctx *ctx, const rps_request *req, int64_t *numerators,
         }
         return RPS_OK;
     } catch (const HipError &e) {
+        return fail(ctx, e);
+    } catch (const TimedOut &e) {
         return fail(ctx, e);
     } catch (const ApiError &e) {
         return fail(ctx, e);
@@ -1050,6 +1124,7 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
     ctx->last_sites = 0.0;
     const auto t_plan = Clock::now();
     try {
+        check_usable(ctx);
         if (!sink || (!requests && n_requests)) {
             throw ApiError{RPS_EINVAL, "null requests or sink"};
         }
@@ -1279,7 +1354,7 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
                                       ctx->t_resid.ptr, ctx->compute));
         }
         RPS_HIP(hipEventRecord(ctx->basis_end, ctx->compute));
-        RPS_HIP(hipEventSynchronize(ctx->basis_end));
+        wait_event(ctx->basis_end, ctx->launch_timeout_ms, "residue-table build");
         guard.built = true;
         float bms = 0.0f;
         RPS_HIP(hipEventElapsedTime(&bms, ctx->basis_start, ctx->basis_end));
@@ -1322,9 +1397,14 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
             RPS_HIP(hipEventRecord(s.k_end, ctx->compute));
             RPS_HIP(hipMemcpyAsync(s.h_state, s.d_state, sizeof(rps::DeviceState),
                                    hipMemcpyDeviceToHost, ctx->compute));
-            RPS_HIP(hipMemcpyAsync(&head, ctx->m_head.ptr, sizeof(head),
+            RPS_HIP(hipMemcpyAsync(ctx->h_head, ctx->m_head.ptr, sizeof(unsigned long long),
                                    hipMemcpyDeviceToHost, ctx->compute));
-            RPS_HIP(hipStreamSynchronize(ctx->compute));
+            RPS_HIP(hipEventRecord(s.ready, ctx->compute));
+            wait_event(s.ready, ctx->launch_timeout_ms,
+                       "many-curve launch " + std::to_string(st.launches + 1) + " (tiles "
+                           + std::to_string(head) + ".." + std::to_string(total_tiles)
+                           + " of " + std::to_string(n) + " requests)");
+            head = *ctx->h_head;
             ++st.launches;
             float kms = 0.0f;
             RPS_HIP(hipEventElapsedTime(&kms, s.k_start, s.k_end));
@@ -1430,6 +1510,8 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
         ctx->last_sites = sites;
         return cancelled ? RPS_ECANCELLED : RPS_OK;
     } catch (const HipError &e) {
+        return fail(ctx, e);
+    } catch (const TimedOut &e) {
         return fail(ctx, e);
     } catch (const ApiError &e) {
         return fail(ctx, e);

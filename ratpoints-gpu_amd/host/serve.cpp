@@ -1,11 +1,28 @@
 // Serving loop over rps_sieve_many (see serve.hpp for the contract).
+//
+// Three kinds of work run at once:
+//   * the scheduling thread owns the device context: it admits checked
+//     requests while slots and the memory budget allow, sieves one round (a
+//     chunk of every active request's denominators in one rps_sieve_many
+//     call) and hands each request's survivors to the worker pool;
+//   * the worker pool verifies those survivors with GMP while the next
+//     round is already on the GPU, and runs the admission checks (request
+//     limits, the squarefree test, the point at infinity) of new requests,
+//     so a slow check never stalls the GPU;
+//   * the scheduling thread applies verified chunks in denominator order per
+//     request and reports finished requests, so results are exactly those of
+//     the same search alone.
 #include "serve.hpp"
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
 #include <deque>
+#include <functional>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -24,29 +41,90 @@ using Clock = std::chrono::steady_clock;
 constexpr double kTableBytes = 36.0 * 512 * 9 * 8;  // residue table per curve
 constexpr double kSurvivorBytes = 16.0;              // num + den + request index
 constexpr double kSurvivorRate = 1.0 / (1ULL << 30); // generous survivors per site
+constexpr int kMaxChunksInFlight = 2;                // sieved, not yet applied
 
 double line_sites(const ServeRequest &r) {
     return 2.0 * static_cast<double>(r.options.numerator_bound) + 1.0;
 }
+
+// A fixed set of threads running posted jobs in order.
+class WorkerPool {
+public:
+    explicit WorkerPool(unsigned n) {
+        for (unsigned i = 0; i < n; ++i) {
+            threads_.emplace_back([this] { loop(); });
+        }
+    }
+    ~WorkerPool() {
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (std::thread &t : threads_) {
+            t.join();
+        }
+    }
+    void post(std::function<void()> job) {
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            jobs_.push_back(std::move(job));
+        }
+        cv_.notify_one();
+    }
+
+private:
+    void loop() {
+        for (;;) {
+            std::function<void()> job;
+            {
+                std::unique_lock<std::mutex> lock(mutex_);
+                cv_.wait(lock, [&] { return stop_ || !jobs_.empty(); });
+                if (jobs_.empty()) {
+                    return;
+                }
+                job = std::move(jobs_.front());
+                jobs_.pop_front();
+            }
+            job();
+        }
+    }
+
+    std::vector<std::thread> threads_;
+    std::mutex mutex_;
+    std::condition_variable cv_;
+    std::deque<std::function<void()>> jobs_;
+    bool stop_ = false;
+};
 
 // One admitted request.
 struct Active {
     ServeRequest request;
     ExactPolynomial polynomial;
     std::vector<const char *> coefficient_ptrs;
-    long long next_t = 0;
+    double submit_ms = 0.0;
+    long long next_t = 0;       // first denominator parameter not yet scheduled
     long long last_t = 0;
-    long long chunk_last = 0;
     double footprint = 0.0;
     ServeResult result;
-    std::vector<long long> num;
-    std::vector<int> den;
+    int next_seq = 0;           // sequence number of the next chunk to schedule
+    int apply_seq = 0;          // sequence number of the next chunk to apply
+    int in_flight = 0;          // chunks scheduled and not yet applied
+    bool complete = false;      // -1 satisfied: nothing more is scheduled
+    bool finished = false;      // reported
+    std::map<int, std::pair<std::vector<PointPair>, double>> verified;  // by sequence
 
-    explicit Active(ServeRequest r)
-        : request(std::move(r)), polynomial(request.options.coefficients) {
+    Active(ServeRequest r, double submitted)
+        : request(std::move(r)), polynomial(request.options.coefficients), submit_ms(submitted) {
         for (const std::string &c : request.options.coefficients) {
             coefficient_ptrs.push_back(c.c_str());
         }
+        next_t = request.options.denominators.first;
+        last_t = request.options.denominators.last;
+    }
+
+    bool schedulable() const {
+        return !complete && next_t <= last_t && in_flight < kMaxChunksInFlight;
     }
 };
 
@@ -61,19 +139,38 @@ rps_request to_request(const Active &a, long long first, long long last) {
     return q;
 }
 
-// Points of one verified chunk, in output order and filtered; returns
-// whether the request is complete (-1 satisfied).
-bool take_points(Active &a, std::vector<PointPair> &found) {
-    order_points(found);
-    for (PointPair &p : found) {
-        if (a.request.options.accepts(p.numerator, p.denominator)) {
-            a.result.points.push_back(std::move(p));
-            if (a.request.first_point) {
-                return true;
-            }
-        }
+// The admission checks of one request: the C ABI's limits, a squarefree f
+// and the point at infinity (src/point_search.cpp:185-189); throws with the
+// message the search alone would give.
+std::shared_ptr<Active> checked_request(ServeRequest request, double submit_ms) {
+    const SearchOptions &opt = request.options;
+    std::vector<const char *> ptrs;
+    for (const std::string &c : opt.coefficients) {
+        ptrs.push_back(c.c_str());
     }
-    return false;
+    rps_request q{};
+    q.coefficients = ptrs.data();
+    q.n_coefficients = static_cast<int32_t>(ptrs.size());
+    q.numerator_bound = opt.numerator_bound;
+    q.den_first = opt.denominators.first;
+    q.den_last = std::max(opt.denominators.first, opt.denominators.last);
+    q.square_denominators = opt.square_denominators ? 1 : 0;
+    char message[256];
+    if (rps_check_request(&q, message, sizeof message) != RPS_OK) {
+        throw std::invalid_argument(message);
+    }
+    if (!ExactPolynomial::is_squarefree(opt.coefficients)) {
+        throw std::invalid_argument("polynomial is not squarefree");
+    }
+    auto a = std::make_shared<Active>(std::move(request), submit_ms);
+    a->result.id = a->request.id;
+    a->result.submit_ms = submit_ms;
+    mpz_class root;
+    if (a->request.options.include_infinity && a->polynomial.square_root(1, 0, root)) {
+        a->result.points.push_back({1, root.get_str(), 0});
+        a->complete = a->request.first_point;
+    }
+    return a;
 }
 
 }  // namespace
@@ -90,185 +187,255 @@ struct Server::Impl {
     ServeCallback on_done;
     Clock::time_point start = Clock::now();
 
+    // Shared with submit(), the workers and the scheduling thread.
     mutable std::mutex mutex;
-    std::condition_variable wake;
-    std::condition_variable idle;
-    std::deque<std::pair<ServeRequest, double>> pending;
+    std::condition_variable wake;   // the scheduling thread has work
+    std::condition_variable idle;   // every submitted request finished
     size_t outstanding = 0;
     bool stopping = false;
     ServeStats totals;
+    size_t checking = 0;                                   // admission checks running
+    std::deque<std::shared_ptr<Active>> ready;             // checked, waiting for a slot
+    std::deque<ServeResult> refused;                       // failed their checks
+    struct Done {
+        std::shared_ptr<Active> active;
+        int seq;
+        std::vector<PointPair> points;
+        double sites;
+    };
+    std::deque<Done> verified;
 
+    // Scheduling thread only.
     rps_ctx *ctx = nullptr;
-    std::vector<std::unique_ptr<Active>> active;
-    double used = 0.0;
     std::string open_error;
-    std::thread worker;
+    std::vector<std::shared_ptr<Active>> active;
+    double used = 0.0;
+
+    std::unique_ptr<WorkerPool> pool;
+    std::thread scheduler;
 
     double now() const {
         return std::chrono::duration<double, std::milli>(Clock::now() - start).count();
     }
 
-    void finish(std::unique_ptr<Active> &a) {
-        a->result.finish_ms = now();
-        used -= a->footprint;
-        on_done(a->result);
-        a.reset();
-        std::lock_guard<std::mutex> lock(mutex);
-        --outstanding;
-        if (outstanding == 0) {
-            idle.notify_all();
-        }
-    }
-
-    void fail_now(ServeRequest &&request, double submit_ms, const std::string &error) {
-        ServeResult r;
-        r.id = request.id;
-        r.error = error;
-        r.submit_ms = submit_ms;
-        r.admit_ms = r.finish_ms = now();
+    void report(ServeResult &r) {
+        r.finish_ms = now();
         on_done(r);
         std::lock_guard<std::mutex> lock(mutex);
-        --outstanding;
-        if (outstanding == 0) {
+        if (--outstanding == 0) {
             idle.notify_all();
         }
     }
 
-    // Moves pending requests onto the device while a slot is free and their
-    // footprint fits (the first always fits an empty device).
-    void admit() {
-        for (;;) {
-            std::pair<ServeRequest, double> next;
+    void finish(const std::shared_ptr<Active> &a) {
+        if (a->finished) {
+            return;
+        }
+        a->finished = true;
+        used -= a->footprint;
+        report(a->result);
+    }
+
+    // Worker side of admission.
+    void check(ServeRequest request, double submit_ms) {
+        std::shared_ptr<Active> a;
+        ServeResult failed;
+        try {
+            a = checked_request(request, submit_ms);
+        } catch (const std::exception &e) {
+            failed.id = request.id;
+            failed.error = e.what();
+            failed.submit_ms = submit_ms;
+        }
+        {
+            std::lock_guard<std::mutex> lock(mutex);
+            --checking;
+            if (a) {
+                ready.push_back(std::move(a));
+            } else {
+                refused.push_back(std::move(failed));
+            }
+        }
+        wake.notify_all();
+    }
+
+    // Worker side of verification: one chunk of one request.
+    void verify(std::shared_ptr<Active> a, int seq, CandidateBatch batch, double sites) {
+        std::vector<PointPair> found;
+        verify_candidates(batch, a->polynomial, found);
+        order_points(found);
+        std::vector<PointPair> kept;
+        for (PointPair &p : found) {
+            if (a->request.options.accepts(p.numerator, p.denominator)) {
+                kept.push_back(std::move(p));
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lock(mutex);
+            verified.push_back({std::move(a), seq, std::move(kept), sites});
+        }
+        wake.notify_all();
+    }
+
+    // Moves checked requests onto the device while a slot is free and their
+    // footprint fits (the first always fits an empty device), in the order
+    // their checks finished.
+    void admit(std::deque<std::shared_ptr<Active>> &incoming) {
+        while (!incoming.empty() && static_cast<int>(active.size()) < options.max_active) {
+            std::shared_ptr<Active> a = incoming.front();
+            const double fp = request_footprint(a->request, options);
+            if (!active.empty() && used + fp > options.memory_budget) {
+                break;
+            }
+            incoming.pop_front();
+            a->result.admit_ms = now();
+            a->footprint = fp;
+            used += fp;
             {
                 std::lock_guard<std::mutex> lock(mutex);
-                if (pending.empty()
-                    || static_cast<int>(active.size()) >= options.max_active) {
-                    return;
-                }
-                const double fp = request_footprint(pending.front().first, options);
-                if (!active.empty() && used + fp > options.memory_budget) {
-                    return;
-                }
-                next = std::move(pending.front());
-                pending.pop_front();
+                ++totals.admitted;
             }
-            ServeRequest &request = next.first;
-            const SearchOptions &opt = request.options;
             if (!open_error.empty()) {
-                fail_now(std::move(request), next.second, open_error);
+                a->result.points.clear();
+                a->result.error = open_error;
+                a->finished = true;
+                used -= fp;
+                report(a->result);
                 continue;
             }
-            std::vector<const char *> ptrs;
-            for (const std::string &c : opt.coefficients) {
-                ptrs.push_back(c.c_str());
-            }
-            rps_request q{};
-            q.coefficients = ptrs.data();
-            q.n_coefficients = static_cast<int32_t>(ptrs.size());
-            q.numerator_bound = opt.numerator_bound;
-            q.den_first = opt.denominators.first;
-            q.den_last = std::max(opt.denominators.first, opt.denominators.last);
-            q.square_denominators = opt.square_denominators ? 1 : 0;
-            char message[256];
-            if (rps_check_request(&q, message, sizeof message) != RPS_OK) {
-                fail_now(std::move(request), next.second, message);
-                continue;
-            }
-            std::unique_ptr<Active> a;
-            try {
-                if (!ExactPolynomial::is_squarefree(opt.coefficients)) {
-                    throw std::invalid_argument("polynomial is not squarefree");
-                }
-                // A copy: `request` must stay whole for fail_now if this throws.
-                a.reset(new Active(request));
-            } catch (const std::exception &e) {
-                fail_now(std::move(request), next.second, e.what());
-                continue;
-            }
-            a->result.id = a->request.id;
-            a->result.submit_ms = next.second;
-            a->result.admit_ms = now();
-            a->next_t = a->request.options.denominators.first;
-            a->last_t = a->request.options.denominators.last;
-            a->footprint = request_footprint(a->request, options);
-            used += a->footprint;
-            ++totals.admitted;
-            // The point at infinity comes first (src/point_search.cpp:143-147).
-            mpz_class root;
-            bool done = a->next_t > a->last_t;
-            if (a->request.options.include_infinity && a->polynomial.square_root(1, 0, root)) {
-                a->result.points.push_back({1, root.get_str(), 0});
-                done = done || a->request.first_point;
-            }
-            active.push_back(std::move(a));
-            if (done) {
-                finish(active.back());
-                active.pop_back();
+            active.push_back(a);
+            if (a->complete || a->next_t > a->last_t) {
+                finish(a);
             }
         }
     }
 
-    // One launch over a chunk of every active request's denominators.
+    // Verified chunks in denominator order per request; a -1 request ends
+    // at its first point and its later chunks are dropped.
+    void apply(std::deque<Done> &done) {
+        for (Done &d : done) {
+            Active &a = *d.active;
+            a.verified.emplace(d.seq, std::make_pair(std::move(d.points), d.sites));
+            for (auto it = a.verified.find(a.apply_seq); it != a.verified.end();
+                 it = a.verified.find(a.apply_seq)) {
+                if (!a.complete) {
+                    a.result.sites += it->second.second;
+                    ++a.result.rounds;
+                    for (PointPair &p : it->second.first) {
+                        a.result.points.push_back(std::move(p));
+                        if (a.request.first_point) {
+                            a.complete = true;
+                            break;
+                        }
+                    }
+                }
+                a.verified.erase(it);
+                ++a.apply_seq;
+                --a.in_flight;
+            }
+            if (a.complete || (a.next_t > a.last_t && a.in_flight == 0)) {
+                finish(d.active);
+            }
+        }
+        prune();
+    }
+
+    // Frees the slots of reported requests with no chunk left in flight.
+    void prune() {
+        active.erase(std::remove_if(active.begin(), active.end(),
+                                    [](const std::shared_ptr<Active> &a) {
+                                        return a->finished && a->in_flight == 0;
+                                    }),
+                     active.end());
+    }
+
+    // One launch over a chunk of every schedulable request.  Shares are
+    // water-filled over the round's sites: requests whose remaining work is
+    // below an equal share get all of it (so short and newly arrived
+    // requests finish in the round they join), the rest split what is left.
     void round() {
-        const size_t n = active.size();
-        // Equal shares: a newcomer's first chunk waits at most one round,
-        // and small requests finish in the round they join.
-        const double share = options.round_sites / static_cast<double>(n);
+        std::vector<std::shared_ptr<Active>> take;
+        for (const auto &a : active) {
+            if (a->schedulable()) {
+                take.push_back(a);
+            }
+        }
+        if (take.empty()) {
+            return;
+        }
+        auto remaining = [](const Active &a) {
+            return line_sites(a.request) * static_cast<double>(a.last_t - a.next_t + 1);
+        };
+        std::sort(take.begin(), take.end(), [&](const auto &x, const auto &y) {
+            return remaining(*x) < remaining(*y);
+        });
+        const size_t n = take.size();
         std::vector<rps_request> reqs(n);
+        std::vector<long long> first(n), last(n);
+        double budget = options.round_sites;
         double sites = 0.0;
         for (size_t i = 0; i < n; ++i) {
-            Active &a = *active[i];
+            Active &a = *take[i];
             const double per_line = line_sites(a.request);
+            const double share = std::max(budget, 0.0) / static_cast<double>(n - i);
             const long long left = a.last_t - a.next_t + 1;
             const long long want = static_cast<long long>(std::floor(share / per_line));
             const long long k = std::max(1LL, std::min(left, want));
-            a.chunk_last = a.next_t + k - 1;
-            reqs[i] = to_request(a, a.next_t, a.chunk_last);
-            a.num.clear();
-            a.den.clear();
+            first[i] = a.next_t;
+            last[i] = a.next_t + k - 1;
+            reqs[i] = to_request(a, first[i], last[i]);
+            budget -= per_line * static_cast<double>(k);
             sites += per_line * static_cast<double>(k);
         }
-        rps_stats st{};
-        auto sink = [](void *user, const int32_t *req, const int64_t *num,
-                       const int32_t *den, size_t count) -> int {
-            auto &act = *static_cast<std::vector<std::unique_ptr<Active>> *>(user);
+        std::vector<CandidateBatch> found(n);
+        auto sink = [](void *user, const int32_t *req, const int64_t *num, const int32_t *den,
+                       size_t count) -> int {
+            auto &out = *static_cast<std::vector<CandidateBatch> *>(user);
             for (size_t i = 0; i < count; ++i) {
-                Active &a = *act[static_cast<size_t>(req[i])];
-                a.num.push_back(num[i]);
-                a.den.push_back(den[i]);
+                CandidateBatch &b = out[static_cast<size_t>(req[i])];
+                b.numerators.push_back(num[i]);
+                b.denominators.push_back(den[i]);
             }
             return 0;
         };
-        const int rc = rps_sieve_many(ctx, reqs.data(), n, sink, &active, &st);
-        ++totals.rounds;
-        totals.sites += sites;
-        totals.kernel_ms += st.sieve_ms;
-        totals.peak_active = std::max<unsigned long long>(totals.peak_active, n);
+        const double t0 = now();
+        rps_stats st{};
+        const int rc = rps_sieve_many(ctx, reqs.data(), n, sink, &found, &st);
+        const double wall = now() - t0;
+        unsigned long long round_no = 0;
+        {
+            std::lock_guard<std::mutex> lock(mutex);
+            round_no = ++totals.rounds;
+            totals.sites += sites;
+            totals.kernel_ms += st.sieve_ms;
+            totals.peak_active = std::max<unsigned long long>(totals.peak_active, active.size());
+        }
+        if (options.progress) {
+            std::fprintf(stderr, "serve: round %llu: %zu requests, %.4g sites, kernel %.2f ms, "
+                         "call %.2f ms, %zu active\n", round_no, n, sites, st.sieve_ms, wall,
+                         active.size());
+        }
         if (rc != RPS_OK) {
             const std::string error = std::string("GPU round failed: ") + rps_last_error(ctx);
             for (auto &a : active) {
+                a->result.points.clear();
                 a->result.error = error;
+                a->complete = true;
                 finish(a);
             }
-            active.clear();
             return;
         }
-        for (auto &a : active) {
-            CandidateBatch batch;
-            batch.numerators = std::move(a->num);
-            batch.denominators = std::move(a->den);
-            std::vector<PointPair> found;
-            verify_candidates(batch, a->polynomial, found);
-            a->result.sites += line_sites(a->request)
-                * static_cast<double>(a->chunk_last - a->next_t + 1);
-            ++a->result.rounds;
-            a->next_t = a->chunk_last + 1;
-            const bool satisfied = take_points(*a, found);
-            if (satisfied || a->next_t > a->last_t) {
-                finish(a);
-            }
+        for (size_t i = 0; i < n; ++i) {
+            std::shared_ptr<Active> a = take[i];
+            const int seq = a->next_seq++;
+            ++a->in_flight;
+            a->next_t = last[i] + 1;
+            const double chunk_sites = line_sites(a->request)
+                * static_cast<double>(last[i] - first[i] + 1);
+            pool->post([this, a, seq, batch = std::move(found[i]), chunk_sites]() mutable {
+                verify(std::move(a), seq, std::move(batch), chunk_sites);
+            });
         }
-        active.erase(std::remove(active.begin(), active.end(), nullptr), active.end());
     }
 
     void run() {
@@ -280,19 +447,52 @@ struct Server::Impl {
             rps_set_option(ctx, "table_slots", std::max(1, options.max_active));
             // Non-primitive (a, b) are never points: drop them on the device.
             rps_set_option(ctx, "primitive_only", 1);
+            rps_set_option(ctx, "launch_timeout_ms", std::max(1LL, options.launch_timeout_ms));
         }
+        std::deque<std::shared_ptr<Active>> incoming;
         for (;;) {
+            std::deque<ServeResult> failed;
+            std::deque<Done> done;
             {
                 std::unique_lock<std::mutex> lock(mutex);
-                wake.wait(lock, [&] { return stopping || !pending.empty() || !active.empty(); });
-                if (stopping && pending.empty() && active.empty()) {
+                auto busy = [&] {
+                    for (const auto &a : active) {
+                        if (a->schedulable()) {
+                            return true;
+                        }
+                    }
+                    return false;
+                };
+                wake.wait(lock, [&] {
+                    return !ready.empty() || !refused.empty() || !verified.empty()
+                        || (!incoming.empty()
+                            && static_cast<int>(active.size()) < options.max_active)
+                        || busy() || (stopping && outstanding == 0 && active.empty());
+                });
+                if (stopping && outstanding == 0 && active.empty()) {
                     break;
                 }
+                for (auto &a : ready) {
+                    incoming.push_back(std::move(a));
+                }
+                ready.clear();
+                failed.swap(refused);
+                done.swap(verified);
             }
-            admit();
-            if (!active.empty()) {
-                round();
+            for (ServeResult &r : failed) {
+                r.admit_ms = now();
+                report(r);
             }
+            apply(done);
+            // Requests that end at admission (-1 with a point at infinity,
+            // an empty range) free their slot at once, so admission can
+            // continue in the same pass.
+            for (size_t before = incoming.size() + 1; incoming.size() < before;) {
+                before = incoming.size();
+                admit(incoming);
+                prune();
+            }
+            round();
         }
         rps_close(ctx);
         ctx = nullptr;
@@ -302,7 +502,9 @@ struct Server::Impl {
 Server::Server(ServeOptions options, ServeCallback on_done) : impl_(new Impl) {
     impl_->options = options;
     impl_->on_done = std::move(on_done);
-    impl_->worker = std::thread([this] { impl_->run(); });
+    const unsigned hw = std::max(2u, std::min(16u, std::thread::hardware_concurrency()));
+    impl_->pool.reset(new WorkerPool(hw));
+    impl_->scheduler = std::thread([this] { impl_->run(); });
 }
 
 Server::~Server() {
@@ -312,16 +514,21 @@ Server::~Server() {
         impl_->stopping = true;
     }
     impl_->wake.notify_all();
-    impl_->worker.join();
+    impl_->scheduler.join();
+    impl_->pool.reset();
 }
 
 void Server::submit(ServeRequest request) {
+    const double at = impl_->now();
     {
         std::lock_guard<std::mutex> lock(impl_->mutex);
-        impl_->pending.emplace_back(std::move(request), impl_->now());
         ++impl_->outstanding;
+        ++impl_->checking;
     }
-    impl_->wake.notify_all();
+    Impl *impl = impl_.get();
+    impl_->pool->post([impl, request = std::move(request), at]() mutable {
+        impl->check(std::move(request), at);
+    });
 }
 
 void Server::drain() {

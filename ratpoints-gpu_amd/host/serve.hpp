@@ -46,6 +46,8 @@ struct ServeOptions {
     int max_active = 256;         // requests on the device at once
     double memory_budget = 8.0 * 1024 * 1024 * 1024;  // device bytes for admitted requests
     int device = 0;
+    bool progress = false;           // one line per round on stderr
+    long long launch_timeout_ms = 300000;  // a round's kernel wait; longer fails the round
 };
 
 struct ServeStats {

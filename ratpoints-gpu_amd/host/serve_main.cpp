@@ -105,7 +105,7 @@ double percentile(std::vector<double> v, double q) {
 void usage() {
     std::cerr << "usage: ratpoints_serve --trace FILE [--mode serve|alone] [--out FILE]\n"
                  "       [--round-sites X] [--max-active N] [--memory-budget BYTES]\n"
-                 "       [--time-scale S]\n";
+                 "       [--time-scale S] [--progress] [--launch-timeout-ms MS]\n";
 }
 
 }  // namespace
@@ -130,6 +130,8 @@ int main(int argc, char **argv) {
         else if (a == "--max-active") options.max_active = std::stoi(value());
         else if (a == "--memory-budget") options.memory_budget = std::stod(value());
         else if (a == "--time-scale") time_scale = std::stod(value());
+        else if (a == "--progress") options.progress = true;
+        else if (a == "--launch-timeout-ms") options.launch_timeout_ms = std::stoll(value());
         else {
             usage();
             return 2;
