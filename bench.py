@@ -51,26 +51,51 @@ def load(name, path):
     return module
 
 
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo), or "unknown"."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    """One thread per host core this process may use, at most 16: the GPU
+    box gives each GPU a share of 16 cores whatever os.cpu_count() shows."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return max(1, min(16, usable))
+
+
 def cpu_baseline(coeffs, gpu_num, gpu_den, gpu_last, seconds_target=15.0):
-    """The oracle's packed-word CPU sieve on the same curve and A, over the
-    first n denominators, n sized for about `seconds_target` of wall time.
-    The oracle's survivors of that sample are also the checker for the timed
-    GPU run over b = 1..gpu_last (n <= gpu_last): its survivors with b <= n
-    must have the same count and digest.
+    """The host-sieve-plan CPU restatement (the oracle's packed-word sieve,
+    denominators split over one thread per usable core) plus the exact stage
+    on its survivors, on the same curve and A over the first n denominators,
+    n sized for about `seconds_target` of wall time.  It is not Stoll's
+    ratpoints, which is not available here.  The oracle's survivors of that
+    sample are also the checker for the timed GPU run over b = 1..gpu_last
+    (n <= gpu_last): its survivors with b <= n must be the same set.
     Returns (baseline, sample_parity)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     lib = oracle.OracleLib()
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
     n = threads
     t0 = time.perf_counter()
     lib.sieve(coeffs, BOUND, 1, n, method="words", threads=threads, collect=False)
     dt = time.perf_counter() - t0
     n = max(threads, int(n * seconds_target / max(dt, 1e-3)))
     n = min(n, LAST, gpu_last)
+    int_coeffs = [int(c) for c in coeffs]
     t0 = time.perf_counter()
-    count, digest, _, _ = lib.sieve(coeffs, BOUND, 1, n, method="words", threads=threads,
-                                    collect=False)
+    count, digest, num, den = lib.sieve(coeffs, BOUND, 1, n, method="words", threads=threads)
+    points = verify_points(int_coeffs, num, den)
     dt = time.perf_counter() - t0
     sites = (2 * BOUND + 1) * n
     baseline = {
@@ -78,8 +103,10 @@ def cpu_baseline(coeffs, gpu_num, gpu_den, gpu_last, seconds_target=15.0):
         "unit": "Gsite/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle packed-word sieve, same curve, A=5e6, b=1..{n} "
-                  f"({sites:.3e} sites, {dt:.1f} s, {count} survivors)",
+        "sample": f"host-sieve-plan CPU restatement, not Stoll's ratpoints: the oracle's "
+                  f"packed-word sieve on {threads} threads (denominators split across them) "
+                  f"plus the exact stage, same curve, A=5e6, b=1..{n} ({sites:.3e} sites, "
+                  f"{dt:.1f} s, {count} survivors, {len(points)} points); CPU {cpu_model()}",
     }
     sel = gpu_den <= n
     gpu_count = int(sel.sum())

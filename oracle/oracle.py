@@ -300,7 +300,8 @@ class OracleLib:
               method="direct", collect=True, cap=None):
         """Returns (count, digest, num, den); num/den sorted by (b, a) when
         collected."""
-        threads = threads or os.cpu_count() or 1
+        # One thread per usable core, at most 16 (a GPU box's CPU share).
+        threads = threads or max(1, min(16, len(os.sched_getaffinity(0))))
         arr = _char_array([str(c) for c in coeffs])
         curve = self.lib.rpo_curve_new(arr, len(coeffs))
         if not curve:
