@@ -94,7 +94,10 @@ def cpu_baseline(coeffs, gpu_num, gpu_den, gpu_last, seconds_target=15.0):
     n = min(n, LAST, gpu_last)
     int_coeffs = [int(c) for c in coeffs]
     t0 = time.perf_counter()
-    count, digest, num, den = lib.sieve(coeffs, BOUND, 1, n, method="words", threads=threads)
+    # A capacity far above the few hundred survivors of this curve, so the
+    # oracle sieves once (without one it counts first, then collects).
+    count, digest, num, den = lib.sieve(coeffs, BOUND, 1, n, method="words", threads=threads,
+                                        cap=1 << 22)
     points = verify_points(int_coeffs, num, den)
     dt = time.perf_counter() - t0
     sites = (2 * BOUND + 1) * n

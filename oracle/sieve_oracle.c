@@ -310,16 +310,18 @@ static void sieve_words_one(const rpo_curve *curve, long long bound,
         }
         unsigned long long entries = words < (unsigned long long)p[k]
             ? words : (unsigned long long)p[k];
+        /* Bit z of `string` is perm[z mod p] for z < p + 64, so the 64
+         * numerators from residue z0 on are bits z0 .. z0 + 63 of it. */
+        uint64_t string[(RPO_MAX_PRIME + 64) / 64 + 2] = {0};
+        for (int z = 0; z < p[k] + 64; ++z) {
+            string[z >> 6] |= (uint64_t)perm[z % p[k]] << (z & 63);
+        }
         for (unsigned long long t = 0; t < entries; ++t) {
-            uint64_t bits = 0;
-            long long z = (long long)((c + 64 * (long long)t) % p[k]);
-            for (int j = 0; j < 64; ++j) {
-                bits |= (uint64_t)perm[z] << j;
-                if (++z == p[k]) {
-                    z = 0;
-                }
-            }
-            rows[k * RPO_MAX_PRIME + t] = bits;
+            const long long z0 = (long long)((c + 64 * (long long)t) % p[k]);
+            const int shift = (int)(z0 & 63);
+            const uint64_t lo = string[z0 >> 6];
+            const uint64_t hi = string[(z0 >> 6) + 1];
+            rows[k * RPO_MAX_PRIME + t] = shift ? (lo >> shift) | (hi << (64 - shift)) : lo;
         }
     }
     int off[RPO_FAST];

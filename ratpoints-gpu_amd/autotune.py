@@ -9,6 +9,8 @@ with ``rps_set_option("variant")``.  Rebuild the library afterwards.
 
     python ratpoints-gpu_amd/autotune.py            # measure and write
     python ratpoints-gpu_amd/autotune.py --dry-run  # measure and print
+    python ratpoints-gpu_amd/autotune.py --run q_u1_f12_w8 --words 4096 [--many]
+                                     # one timed sieve of one variant (profiling)
 """
 
 import argparse
@@ -75,11 +77,48 @@ def write_table(rows, path=TABLE):
         fh.write("\n".join(lines) + "\n")
 
 
+def run_one(variant, words, lines, many):
+    """One timed sieve of `lines` lines of `words` words on the planted
+    sextic with one variant, on the plain path or (many) as 64 requests of
+    equal shape in one rps_sieve_many call: the workload a profiler wraps."""
+    pkg = load("ratpoints_gpu_amd", os.path.join(HERE, "__init__.py"))
+    wl = load("ratpoints_gpu_amd_workloads", os.path.join(HERE, "workloads.py"))
+    coeffs = [str(c) for c in wl.sextic_a5e6_curve()[0]]
+    bound = 32 * words - 32
+    with pkg.SieveContext(0) as ctx:
+        ctx.set_option("variant", pkg.variants().index(variant))
+        if many:
+            per = max(1, lines // 64)
+            reqs = [{"coefficients": coeffs, "bound": bound, "first": 1 + k * per,
+                     "last": (k + 1) * per} for k in range(64)]
+            ctx.run_many(reqs[:2], lambda q, n, d: False)
+            t0 = time.perf_counter()
+            ctx.run_many(reqs, lambda q, n, d: False)
+        else:
+            ctx.run(coeffs, bound, 1, 65536, False, lambda n, d: False)
+            t0 = time.perf_counter()
+            ctx.run(coeffs, bound, 1, lines, False, lambda n, d: False)
+        wall = time.perf_counter() - t0
+        ms, launches, sites = ctx.kernel_timing()
+    print(json.dumps({"variant": variant, "words": words, "lines": lines, "many": many,
+                      "kernel_ms": round(ms, 3), "launches": launches, "sites": sites,
+                      "kernel_gsites": round(sites / ms / 1e6, 1),
+                      "wall_gsites": round(sites / wall / 1e9, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--dry-run", action="store_true")
     ap.add_argument("--out", default=TABLE, help="table path (default: csrc/tuning_table.inc)")
+    ap.add_argument("--run", metavar="VARIANT",
+                    help="run one variant at --words (for profilers) instead of tuning")
+    ap.add_argument("--words", type=int, default=4096)
+    ap.add_argument("--lines", type=int, default=262144)
+    ap.add_argument("--many", action="store_true", help="--run through rps_sieve_many")
     args = ap.parse_args()
+    if args.run:
+        run_one(args.run, args.words, args.lines, args.many)
+        return
     rows = measure()
     if not args.dry_run:
         write_table(rows, args.out)

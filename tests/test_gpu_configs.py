@@ -66,10 +66,10 @@ def test_batched_4096_curves_match_oracle(fresh_gpu, oracle_lib, workloads):
     assert not missing, missing[:10]
     print("planted points: all present", flush=True)
 
-    # The oracle costs about 2.5 ms per denominator line on one thread, so
-    # it checks every curve's first 8 lines and the whole box of the curves
-    # it finishes cheapest (about 600 thread-seconds in all), one curve per
-    # thread of a 16-thread pool.
+    # On one thread the oracle costs about 0.3 ms per denominator line plus
+    # 0.5 ns per site, so it checks every curve's first 8 lines and the
+    # whole box of the curves it finishes cheapest (about 600 thread-seconds
+    # in all, 751 curves), one curve per thread of a 16-thread pool.
     def first_lines(i):
         c = curves[i]
         last = min(c["last"], c["first"] + 7)
@@ -87,7 +87,7 @@ def test_batched_4096_curves_match_oracle(fresh_gpu, oracle_lib, workloads):
     def cost(i):
         c = curves[i]
         lines = c["last"] - c["first"] + 1
-        return lines * 2.5e-3 + (2 * c["bound"] + 1) * lines * 5e-10
+        return lines * 3e-4 + (2 * c["bound"] + 1) * lines * 5e-10
 
     order = sorted(range(len(curves)), key=cost)
     whole, budget = [], 600.0
@@ -108,4 +108,4 @@ def test_batched_4096_curves_match_oracle(fresh_gpu, oracle_lib, workloads):
         bad = [i for i, ok in pool.map(whole_box, whole) if not ok]
     assert not bad, bad[:10]
     print(f"whole boxes of {len(whole)} curves: identical", flush=True)
-    assert len(whole) >= 150
+    assert len(whole) >= 700
