@@ -9,8 +9,9 @@ with ``rps_set_option("variant")``.  Rebuild the library afterwards.
 
     python ratpoints-gpu_amd/autotune.py            # measure and write
     python ratpoints-gpu_amd/autotune.py --dry-run  # measure and print
-    python ratpoints-gpu_amd/autotune.py --run q_u1_f12_w8 --words 4096 [--many]
-                                     # one timed sieve of one variant (profiling)
+    python ratpoints-gpu_amd/autotune.py --run q_u1_f12_w8 --words 4096 [--many] [--lib SO]
+                                     # one timed sieve of one variant (profiling,
+                                     # or comparing another build of the library)
 """
 
 import argparse
@@ -77,16 +78,19 @@ def write_table(rows, path=TABLE):
         fh.write("\n".join(lines) + "\n")
 
 
-def run_one(variant, words, lines, many):
+def run_one(variant, words, lines, many, lib=None):
     """One timed sieve of `lines` lines of `words` words on the planted
     sextic with one variant, on the plain path or (many) as 64 requests of
-    equal shape in one rps_sieve_many call: the workload a profiler wraps."""
+    equal shape in one rps_sieve_many call: the workload a profiler wraps.
+    `lib`: another build of the library (same C ABI) to time instead."""
     pkg = load("ratpoints_gpu_amd", os.path.join(HERE, "__init__.py"))
     wl = load("ratpoints_gpu_amd_workloads", os.path.join(HERE, "workloads.py"))
     coeffs = [str(c) for c in wl.sextic_a5e6_curve()[0]]
     bound = 32 * words - 32
-    with pkg.SieveContext(0) as ctx:
-        ctx.set_option("variant", pkg.variants().index(variant))
+    handle = pkg.bind_library(lib) if lib else None
+    with pkg.SieveContext(0, lib=handle) as ctx:
+        names = [ctx.lib.rps_variant_name(i).decode() for i in range(ctx.lib.rps_variant_count())]
+        ctx.set_option("variant", names.index(variant))
         if many:
             per = max(1, lines // 64)
             reqs = [{"coefficients": coeffs, "bound": bound, "first": 1 + k * per,
@@ -101,6 +105,7 @@ def run_one(variant, words, lines, many):
         wall = time.perf_counter() - t0
         ms, launches, sites = ctx.kernel_timing()
     print(json.dumps({"variant": variant, "words": words, "lines": lines, "many": many,
+                      "lib": lib or "default",
                       "kernel_ms": round(ms, 3), "launches": launches, "sites": sites,
                       "kernel_gsites": round(sites / ms / 1e6, 1),
                       "wall_gsites": round(sites / wall / 1e9, 1)}), flush=True)
@@ -115,9 +120,10 @@ def main():
     ap.add_argument("--words", type=int, default=4096)
     ap.add_argument("--lines", type=int, default=262144)
     ap.add_argument("--many", action="store_true", help="--run through rps_sieve_many")
+    ap.add_argument("--lib", help="--run on another build of the library")
     args = ap.parse_args()
     if args.run:
-        run_one(args.run, args.words, args.lines, args.many)
+        run_one(args.run, args.words, args.lines, args.many, args.lib)
         return
     rows = measure()
     if not args.dry_run:

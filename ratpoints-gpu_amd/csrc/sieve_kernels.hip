@@ -124,7 +124,7 @@ __device__ __forceinline__ uint64_t primitive_bits(uint64_t m, uint64_t a0, int 
 }
 
 // Wave-level survivor emission.  Lane l holds N masks; mask k covers the 64
-// numerators of word word[k] on denominator b.  A prefix sum of per-lane
+// numerators of word word[k] on denominator den[k].  A prefix sum of per-lane
 // counts gives every lane its slots under one atomic reservation per wave,
 // all or nothing: when the reservation does not fit, first_fail records its
 // start, nothing is written and false is returned, so the wave can stop and
@@ -133,7 +133,7 @@ __device__ __forceinline__ uint64_t primitive_bits(uint64_t m, uint64_t a0, int 
 template <int N>
 __device__ __forceinline__ bool emit_words(const OutDesc &O, long long bound,
                                            const uint64_t (&m_in)[N],
-                                           const uint64_t (&word)[N], int b,
+                                           const uint64_t (&word)[N], const int (&den)[N],
                                            int request, int lane) {
     uint64_t m[N];
 #pragma unroll
@@ -144,7 +144,7 @@ __device__ __forceinline__ bool emit_words(const OutDesc &O, long long bound,
         const uint64_t neg_a = static_cast<uint64_t>(-bound);
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-            m[k] = m[k] ? primitive_bits(m[k], neg_a + 64 * word[k], b) : 0ULL;
+            m[k] = m[k] ? primitive_bits(m[k], neg_a + 64 * word[k], den[k]) : 0ULL;
         }
     }
     uint32_t cnt = 0;
@@ -186,7 +186,7 @@ __device__ __forceinline__ bool emit_words(const OutDesc &O, long long bound,
             const int j = __ffsll(static_cast<long long>(mm)) - 1;
             mm &= mm - 1;
             O.num[pos] = static_cast<long long>(a0 + j);
-            O.den[pos] = b;
+            O.den[pos] = den[k];
             if (O.req) {
                 O.req[pos] = request;
             }
@@ -203,14 +203,23 @@ __device__ __forceinline__ bool emit_survivors(const OutDesc &O, long long bound
                                                uint64_t first_word, int b,
                                                int request, int lane) {
     uint64_t word[U];
+    int den[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         word[u] = first_word + 64 * u;
+        den[u] = b;
     }
-    return emit_words<U>(O, bound, m, word, b, request, lane);
+    return emit_words<U>(O, bound, m, word, den, request, lane);
 }
 
 }  // namespace
+
+// Where a wave stops when survivors do not fit: the line of its run and the
+// position within it (a word, or W + u for the queued tiles).
+struct Cursor {
+    uint32_t line;
+    uint64_t pos;
+};
 
 // ---------------------------------------------------------------------------
 // Row tile: one denominator, one chunk [ts, te) of its line.
@@ -278,20 +287,31 @@ struct RowTile {
     // Entries computed per row (a wide element also holds entry e + 64),
     // rounded up to whole waves.
     static constexpr int kEntries = WIDE ? (slot_elems(0) + 64 + 63) / 64 * 64 : kSlot;
-    static constexpr int kLate = kSelected - FAST;
-    // LDS layout, in 64-bit words: rows, residue strings, then six arrays of
-    // 32 uint32 (p, (-A) mod p, b mod p, candidate index, ts mod p, magic).
+    static constexpr int kFast = FAST;
+    // LDS layout, in 64-bit words: rows, the dense primes' residue strings,
+    // then two sets (by line parity) of the per-line arrays: six arrays of
+    // 32 uint32 (p, (-A) mod p, b mod p, candidate index, ts mod p, magic)
+    // and a flag, 1 when the line's dense primes are the FAST largest
+    // candidates.
     static constexpr int kCext = row_base(FAST);
-    static constexpr int kSmall = kCext + kSelected * 10;
-    static constexpr int kLdsWords = kSmall + 3 * kSelected;
+    static constexpr int kSmall = kCext + FAST * 10;
+    static constexpr int kSmallWords = 3 * kSelected + 1;
+    static constexpr int kDefaultFlag = 6 * kSelected;  // u32 index in a set
+    static constexpr int kLdsWords = kSmall + 2 * kSmallWords;
     static_assert(kStride < 293, "a row step must stay below every prime");
     static_assert(FAST >= 1 && FAST <= kSelected, "dense prime count");
-    static_assert(kLate <= 32, "late primes must fit a half wave");
     static_assert(!WIDE || U == 2, "wide rows pair the two words of U = 2");
 
+    __device__ __forceinline__ static uint32_t *small(uint64_t *pool, int par) {
+        return reinterpret_cast<uint32_t *>(pool + kSmall + par * kSmallWords);
+    }
+    __device__ __forceinline__ static const uint32_t *small(const uint64_t *pool, int par) {
+        return reinterpret_cast<const uint32_t *>(pool + kSmall + par * kSmallWords);
+    }
+
     // Word range of wave `wave` in [ts, te), in whole strides.
-    __device__ static void wave_range(uint64_t ts, uint64_t te, int wave,
-                                      uint64_t &ws, uint64_t &we) {
+    __device__ __forceinline__ static void wave_range(uint64_t ts, uint64_t te, int wave,
+                                                      uint64_t &ws, uint64_t &we) {
         const uint64_t per =
             ((te - ts + static_cast<uint64_t>(kStride) * WAVES - 1)
              / (static_cast<uint64_t>(kStride) * WAVES)) * kStride;
@@ -306,8 +326,8 @@ struct RowTile {
     // word e of the line mod p, the window of the residue string at
     // (-A + 64 e) mod p; entries e >= p repeat the row so that reads never
     // wrap.
-    __device__ static void build_rows(uint64_t *pool, int tid) {
-        const uint32_t *s_p = reinterpret_cast<const uint32_t *>(pool + kSmall);
+    __device__ __forceinline__ static void build_rows(uint64_t *pool, int tid, int par) {
+        const uint32_t *s_p = small(pool, par);
         const uint32_t *s_c = s_p + kSelected;
         const uint32_t *s_magic = s_p + 5 * kSelected;
         const uint32_t *cext = reinterpret_cast<const uint32_t *>(pool + kCext);
@@ -344,9 +364,10 @@ struct RowTile {
     }
 
     // Entry e of every dense row.
-    __device__ static void build_entries(uint64_t *pool, const uint32_t *cext,
-                                         const uint32_t (&P)[FAST], const uint32_t (&C)[FAST],
-                                         const uint32_t (&M)[FAST], uint32_t e) {
+    __device__ __forceinline__ static void build_entries(uint64_t *pool, const uint32_t *cext,
+                                                         const uint32_t (&P)[FAST],
+                                                         const uint32_t (&C)[FAST],
+                                                         const uint32_t (&M)[FAST], uint32_t e) {
         uint32_t o[FAST];
 #pragma unroll
         for (int s = 0; s < FAST; ++s) {
@@ -381,6 +402,112 @@ struct RowTile {
         }
     }
 
+    // Lines whose dense primes are the FAST largest candidates (none of
+    // them divides b: nearly every line) have dense rows whose entry e is
+    // the window at o = (c_s + 64 e) mod p_s of the line's string, and that
+    // offset depends on the curve only.  A Plan holds, per thread, the LDS
+    // byte address of the string word o / 32 shifted left by 5 with o mod 32
+    // in the low bits (the funnel shift reads those), for each entry the
+    // thread builds: entry tid of every row, and its share of the tail
+    // entries kThreads <= e < kEntries, which come in columns of 64 whose
+    // FAST x kTailCols (row, column) units are dealt to the waves in turn.
+    // Such a line's rows cost three LDS reads and two funnel shifts per
+    // entry.  The plan stays in registers through the sweep, so only the
+    // one-word tiles (U = 1), whose sweep leaves room, carry one; the
+    // others build every line with the reductions.
+    static constexpr bool kPlanned = !WIDE && U == 1;
+    static constexpr int kTailCols = kEntries > kThreads ? (kEntries - kThreads) / 64 : 0;
+    static constexpr int kTailUnits = FAST * kTailCols;
+    static constexpr int kTailPerWave = (kTailUnits + WAVES - 1) / WAVES;
+    struct Plan {
+        uint32_t main[kPlanned ? FAST : 1];
+        uint32_t tail[kPlanned && kTailPerWave > 0 ? kTailPerWave : 1];
+    };
+
+    // Packed geometry of entry e of row s (prime p, c = (-A) mod p).
+    __device__ __forceinline__ static uint32_t plan_entry(int s, uint32_t p, uint32_t magic,
+                                                          uint32_t c, uint32_t e) {
+        const uint32_t o = mod_small(c + 64u * e, p, magic);
+        const uint32_t byte = static_cast<uint32_t>(kCext * 8 + s * 80) + (o >> 5) * 4u;
+        return (byte << 5) | (o & 31u);
+    }
+
+    // The plan of a curve from its candidate arrays (u32 p, magic, c of the
+    // 36 candidates, ascending: rank s of a default line is candidate 35 - s).
+    __device__ __forceinline__ static Plan make_plan(const uint32_t *cp, int tid, int lane,
+                                                     int wave) {
+        Plan pl{};
+        if (kPlanned) {
+#pragma unroll
+            for (int s = 0; s < FAST; ++s) {
+                const int k = kCandidates - 1 - s;
+                pl.main[s] = plan_entry(s, cp[k], cp[kCandidates + k], cp[2 * kCandidates + k],
+                                        static_cast<uint32_t>(tid));
+            }
+#pragma unroll
+            for (int i = 0; i < kTailPerWave; ++i) {
+                const int j = wave + WAVES * i;
+                if (j < kTailUnits) {
+                    const int s = j % FAST;
+                    const int k = kCandidates - 1 - s;
+                    const uint32_t e = static_cast<uint32_t>(kThreads + 64 * (j / FAST) + lane);
+                    pl.tail[i] = plan_entry(s, cp[k], cp[kCandidates + k],
+                                            cp[2 * kCandidates + k], e);
+                }
+            }
+        }
+        return pl;
+    }
+
+    // The window a plan entry names.
+    __device__ __forceinline__ static uint64_t plan_window(const uint64_t *pool, uint32_t g) {
+        const uint32_t *d =
+            reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(pool) + (g >> 5));
+        const uint32_t d0 = d[0];
+        const uint32_t d1 = d[1];
+        const uint32_t d2 = d[2];
+        const uint32_t lo = __builtin_amdgcn_alignbit(d1, d0, g & 31u);
+        const uint32_t hi = __builtin_amdgcn_alignbit(d2, d1, g & 31u);
+        return (static_cast<uint64_t>(hi) << 32) | lo;
+    }
+
+    // Dense rows of a line with the default selection, from the plan.
+    __device__ __forceinline__ static void build_planned(uint64_t *pool, const Plan &pl, int tid,
+                                                         int lane, int wave) {
+        if (!kPlanned) {
+            return;
+        }
+        if (tid < kEntries) {
+            // Four rows' reads in flight at a time keeps the register peak
+            // of the set-up below the sweep's.
+#pragma unroll
+            for (int s0 = 0; s0 < FAST; s0 += 4) {
+                uint64_t w[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (s0 + j < FAST) {
+                        w[j] = plan_window(pool, pl.main[s0 + j]);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (s0 + j < FAST) {
+                        pool[row_base(s0 + j) + tid] = w[j];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kTailPerWave; ++i) {
+            const int j = wave + WAVES * i;
+            if (j < kTailUnits) {
+                const int s = j % FAST;
+                const int e = kThreads + 64 * (j / FAST) + lane;
+                pool[s * kSlot + e] = plan_window(pool, pl.tail[i]);
+            }
+        }
+    }
+
     // Row offsets of a wave: lane s < FAST holds dense prime s and the
     // offset W mod p_s of the wave's next iteration; the other lanes hold 0
     // and p = kStride, so their offset stays 0.  The first kScalarRows rows
@@ -400,8 +527,10 @@ struct RowTile {
 
     // Lane s < FAST gets prime s and the offset of word `rel` (from the tile
     // start) in its row.
-    __device__ static Offsets row_offsets(const uint32_t *s_p, const uint32_t *s_base,
-                                          const uint32_t *s_magic, uint32_t rel, int lane) {
+    __device__ __forceinline__ static Offsets row_offsets(const uint32_t *s_p,
+                                                          const uint32_t *s_base,
+                                                          const uint32_t *s_magic, uint32_t rel,
+                                                          int lane) {
         Offsets o{};
         o.off = 0u;
         o.p = static_cast<uint32_t>(kStride);
@@ -427,8 +556,8 @@ struct RowTile {
     // and advanced by three vector instructions for all rows, instead of
     // three scalar instructions per row, which made the scalar unit the
     // limit of the loop; the first kScalarRows rows use their SGPR copies.
-    __device__ static void dense(const uint64_t *pool, Offsets &o, int lane,
-                                 uint64_t (&m)[U]) {
+    __device__ __forceinline__ static void dense(const uint64_t *pool, Offsets &o, int lane,
+                                                 uint64_t (&m)[U]) {
         uint32_t off[FAST];
 #pragma unroll
         for (int s = 0; s < FAST; ++s) {
@@ -492,112 +621,6 @@ struct RowTile {
             m[u] = (static_cast<uint64_t>(hi[u]) << 32) | lo[u];
         }
     }
-
-    // One wave sweeps [ws, we).  Returns we when done, or the start of the
-    // iteration whose survivors did not fit the buffer.
-    __device__ static uint64_t sweep(uint64_t *pool, int b, uint64_t ts,
-                                     uint64_t ws, uint64_t we, int /*wave*/,
-                                     const LineDesc &L, const OutDesc &O,
-                                     int request, int lane) {
-        const uint32_t *s_p = reinterpret_cast<const uint32_t *>(pool + kSmall);
-        const uint32_t *s_c = s_p + kSelected;
-        const uint32_t *s_base = s_c + 3 * kSelected;
-        const uint32_t *s_magic = s_base + kSelected;
-        const uint64_t *cext = pool + kCext;
-        // Row offsets: word ws mod p for each dense prime.
-        Offsets o = row_offsets(s_p, s_base, s_magic, static_cast<uint32_t>(ws - ts), lane);
-        // Late-prime constants: lane j of each 32-lane half owns prime
-        // FAST + j.
-        const int lj = lane & 31;
-        const bool late_lane = lj < kLate;
-        uint32_t lp = 1, lmagic = 0, lbase = 0, lc = 0;
-        const uint64_t *lcx = cext;
-        if (late_lane) {
-            lp = s_p[FAST + lj];
-            lmagic = s_magic[FAST + lj];
-            lbase = s_base[FAST + lj];
-            lc = s_c[FAST + lj];
-            lcx = cext + (FAST + lj) * 10;
-        }
-
-        const uint64_t last_word = L.words - 1;
-        uint64_t W = ws;
-        for (; W < we; W += kStride) {
-            uint64_t m[U];
-            dense(pool, o, lane, m);
-            if (W + kStride >= we) {
-                // Last iteration of this wave: drop words past the wave's
-                // range and the sites past +A in the line's final word.
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint64_t w = W + lane + 64 * u;
-                    if (w >= we) {
-                        m[u] = 0;
-                    } else if (w == last_word) {
-                        m[u] &= L.last_mask;
-                    }
-                }
-            }
-            uint64_t any = 0;
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                any |= m[u];
-            }
-            if (!__ballot(any != 0)) {
-                continue;
-            }
-            // Remaining primes, only for words that still have bits set.  Up
-            // to two such words are taken per pass as wave-uniform values; in
-            // each 32-lane half, lane j evaluates late prime FAST + j for its
-            // word and the half ANDs the row words together.
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                uint64_t pend = __ballot(m[u] != 0);
-                while (pend) {
-                    const int l0 = __builtin_ctzll(pend);
-                    pend &= pend - 1;
-                    int l1 = -1;
-                    if (pend) {
-                        l1 = __builtin_ctzll(pend);
-                        pend &= pend - 1;
-                    }
-                    const int src = lane < 32 ? l0 : l1;
-                    uint64_t e = ~0ULL;
-                    if (late_lane && src >= 0) {
-                        const uint32_t rel =
-                            static_cast<uint32_t>(W - ts) + src + 64u * u;
-                        uint32_t wm = mod_small(rel, lp, lmagic) + lbase;
-                        wm = wm >= lp ? wm - lp : wm;
-                        e = window64(lcx, mod_small(lc + 64u * wm, lp, lmagic));
-                    }
-#pragma unroll
-                    for (int d = 1; d < 32; d <<= 1) {
-                        e &= __shfl_xor(e, d, 64);
-                    }
-                    const uint64_t r0 = readlane64(e, 0);
-                    const uint64_t r1 = readlane64(e, 32);
-                    if (lane == l0) {
-                        m[u] &= r0;
-                    }
-                    if (lane == l1) {
-                        m[u] &= r1;
-                    }
-                }
-            }
-            uint64_t live = 0;
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                live |= m[u];
-            }
-            if (!__ballot(live != 0)) {
-                continue;
-            }
-            if (!emit_survivors<U>(O, L.bound, m, W + lane, b, request, lane)) {
-                return W;
-            }
-        }
-        return we;
-    }
 };
 
 // ---------------------------------------------------------------------------
@@ -607,8 +630,21 @@ struct RowTile {
 // queued the wave drains them with every lane busy -- up to two entries per
 // lane, each late prime evaluated from the residue string, a wave-uniform
 // exit once all entries are empty -- and commits all of them in one
-// reservation.  The resume cursor is W + u (iteration W, sub-word u), the
-// oldest uncommitted queue entry, so nothing is emitted twice.
+// reservation.
+//
+// The queue carries over from one line of a run to the next: the late
+// primes' strings, constants and the denominator of each line go into a
+// slot of a ring of kRing line slots in LDS, and a queue entry names its
+// slot, so one drain can hold the words of several lines.  A wave drains
+// early only before a line would reuse the slot of its oldest entry, and
+// once at the end of the run.  On a line of a few thousand words each wave
+// queues only a few words per line; draining them per line cost as much
+// as the dense sweep.
+//
+// The resume cursor is (line of the oldest uncommitted entry, W + u of that
+// entry): a drain commits the whole queue or nothing, so every entry after
+// the oldest uncommitted one is uncommitted too, and a sweep that restarts
+// there queues exactly those entries again.
 // ---------------------------------------------------------------------------
 template <int U, int FAST, int WAVES, bool WIDE = false>
 struct RowTileQ {
@@ -616,47 +652,111 @@ struct RowTileQ {
     static constexpr int kWaves = WAVES;
     static constexpr int kStride = Base::kStride;
     static constexpr int kSlot = Base::kSlot;
+    static constexpr int kLate = kSelected - FAST;
     static constexpr int kQueue = 128;
     static constexpr int kGroup = 3;  // late primes per drain step
-    // Per wave: 128 masks (128 words) and 128 uint32 word offsets (64 words).
+    // Per wave: 128 masks (128 words) and 128 uint32 entry tags (64 words).
     static constexpr int kQueueWords = kQueue + kQueue / 2;
     static constexpr int kQueueBase = Base::kLdsWords;
-    static constexpr int kLdsWords = kQueueBase + WAVES * kQueueWords;
+    // Line slot, in 64-bit words: late strings [kLate][10], constants
+    // [kLate] (p | cb << 16 | magic << 32, cb = the string offset of the
+    // tile's first word), the denominator.
+    static constexpr int kRingConst = kLate * 10;
+    static constexpr int kRingDen = kRingConst + kLate;
+    static constexpr int kRingSlotWords = kRingDen + 1;
+    static constexpr int kRingBase = kQueueBase + WAVES * kQueueWords;
+    // As many slots (up to 8) as keep the workgroups per CU that the tile
+    // had without the ring: 160 KiB of LDS per CU, less the run's own arrays
+    // and the kernels' scalars (kRunExtra words).
+    static constexpr int kRunExtra = 96;
+    static constexpr int kTwoPerCu = 10200;
+    static constexpr int kOnePerCu = 20400;
+    static constexpr int kBudget =
+        kRingBase + kRunExtra + kRingSlotWords <= kTwoPerCu ? kTwoPerCu : kOnePerCu;
+    static constexpr int kRingFit = (kBudget - kRingBase - kRunExtra) / kRingSlotWords;
+    static constexpr int kRing = kRingFit < 1 ? 1 : (kRingFit > 8 ? 8 : kRingFit);
+    static constexpr int kLdsWords = kRingBase + kRing * kRingSlotWords;
+    // Entry tag: word offset from the tile start below bit 26 (tiles of
+    // several lines are whole lines, and lines stay below 2^25 words), ring
+    // slot from bit 28.
+    static constexpr int kSlotShift = 28;
+    static constexpr uint32_t kRelMask = (1u << 26) - 1;
     static_assert((kStride & (kStride - 1)) == 0, "the cursor encodes u below the stride");
+    static_assert(kLate >= 1 && kLate <= 32, "late primes");
 
-    __device__ static void wave_range(uint64_t ts, uint64_t te, int wave,
-                                      uint64_t &ws, uint64_t &we) {
+    __device__ __forceinline__ static void wave_range(uint64_t ts, uint64_t te, int wave,
+                                                      uint64_t &ws, uint64_t &we) {
         Base::wave_range(ts, te, wave, ws, we);
+    }
+
+    __device__ __forceinline__ static uint64_t *ring(uint64_t *pool, int slot) {
+        return pool + kRingBase + slot * kRingSlotWords;
+    }
+    __device__ __forceinline__ static const uint64_t *ring(const uint64_t *pool, uint32_t slot) {
+        return pool + kRingBase + slot * kRingSlotWords;
+    }
+
+    // Where word 0 of string s (selection rank) of the line in `slot` goes.
+    __device__ __forceinline__ static uint64_t *string(uint64_t *pool, int s, int slot) {
+        return s < FAST ? pool + Base::kCext + s * 10 : ring(pool, slot) + (s - FAST) * 10;
+    }
+
+    // The line in `slot` selected prime p (its magic) at `rank`, with string
+    // offset cb at the tile's first word; denominator b.
+    __device__ __forceinline__ static void note_line(uint64_t *pool, int slot, int rank, uint32_t p,
+                                                     uint32_t magic, uint32_t cb, uint32_t b) {
+        uint64_t *r = ring(pool, slot);
+        if (rank >= FAST) {
+            r[kRingConst + rank - FAST] =
+                (static_cast<uint64_t>(magic) << 32) | p | (cb << 16);
+        }
+        if (rank == 0) {
+            r[kRingDen] = b;
+        }
+    }
+
+    // A wave's queue between lines: entries waiting and the run line of the
+    // oldest.
+    struct Carry {
+        uint32_t queued = 0;
+        uint32_t first_line = 0;
+    };
+
+    __device__ __forceinline__ static uint64_t *queue_masks(uint64_t *pool, int wave) {
+        return pool + kQueueBase + wave * kQueueWords;
+    }
+    __device__ __forceinline__ static const uint64_t *queue_masks(const uint64_t *pool, int wave) {
+        return pool + kQueueBase + wave * kQueueWords;
     }
 
     // Applies the late primes to the n queued entries (n <= 128) and emits
     // the survivors; false when they did not fit the buffer.  Entry word
-    // ts + rel has offset (cb + 64 rel) mod p in the residue string (rel <
-    // 2^25, so the sum stays below 2^32), and its window is two funnel
-    // shifts of three 32-bit string words.
-    // Lane j of (lp, lmagic, lcb) holds late prime FAST + j: p, its magic and
-    // its line offset (c + 64 (ts mod p)) mod p.
-    __device__ static bool drain(const uint64_t *pool, const uint64_t *qm,
-                                 const uint32_t *qw, uint32_t lp, uint32_t lmagic,
-                                 uint32_t lcb, uint32_t n, int b, uint64_t ts,
-                                 const LineDesc &L, const OutDesc &O,
-                                 int request, int lane) {
-        const uint32_t *cext = reinterpret_cast<const uint32_t *>(pool + Base::kCext);
+    // ts + rel of the line in slot k has offset (cb + 64 rel) mod p in that
+    // line's string (rel < 2^26, so the sum stays below 2^32), and its
+    // window is two funnel shifts of three 32-bit string words.
+    __device__ __forceinline__ static bool drain(const uint64_t *pool, int wave, uint32_t n,
+                                                 uint64_t ts, const LineDesc &L, const OutDesc &O,
+                                                 int request, int lane) {
+        const uint64_t *qm = queue_masks(pool, wave);
+        const uint32_t *qw = reinterpret_cast<const uint32_t *>(qm + kQueue);
         uint64_t m[2];
-        uint32_t rel[2];
+        uint32_t rel6[2];
+        uint32_t rs[2];  // first word of the entry's ring slot, from the ring base
         wave_sync();
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const uint32_t idx = lane + 64u * k;
             const bool valid = idx < n;
+            const uint32_t tag = valid ? qw[idx] : 0u;
             m[k] = valid ? qm[idx] : 0ULL;
-            rel[k] = valid ? qw[idx] << 6 : 0u;
+            rel6[k] = (tag & kRelMask) << 6;
+            rs[k] = (tag >> kSlotShift) * static_cast<uint32_t>(kRingSlotWords);
         }
+        const uint64_t *rb = pool + kRingBase;
         wave_sync();
-        // Late primes in groups of kGroup: the group's string reads are all
-        // in flight together, and the wave-uniform exit is tested once per
-        // group (a drain typically ends within a few primes, so one LDS
-        // round trip per group instead of per prime).
+        // Late primes in groups of kGroup: the group's reads are all in
+        // flight together, and the wave-uniform exit is tested once per group
+        // (a drain typically ends within a few primes).
         for (int s0 = FAST; s0 < kSelected; s0 += kGroup) {
             if (!__ballot((m[0] | m[1]) != 0)) {
                 break;
@@ -668,15 +768,16 @@ struct RowTileQ {
                 win[j][0] = ~0ULL;
                 win[j][1] = ~0ULL;
                 if (s < kSelected) {
-                    const uint32_t p = __builtin_amdgcn_readlane(lp, s - FAST);
-                    const uint32_t magic = __builtin_amdgcn_readlane(lmagic, s - FAST);
-                    const uint32_t cb = __builtin_amdgcn_readlane(lcb, s - FAST);
-                    const uint32_t *cx = cext + s * 20;
 #pragma unroll
                     for (int k = 0; k < 2; ++k) {
                         if (m[k]) {
-                            const uint32_t o = mod_small(cb + rel[k], p, magic);
-                            const uint32_t *d = cx + (o >> 5);
+                            const uint64_t cst = rb[rs[k] + kRingConst + s - FAST];
+                            const uint32_t pc = static_cast<uint32_t>(cst);
+                            const uint32_t o = mod_small((pc >> 16) + rel6[k], pc & 0xffffu,
+                                                         static_cast<uint32_t>(cst >> 32));
+                            const uint32_t *d =
+                                reinterpret_cast<const uint32_t *>(rb + rs[k] + (s - FAST) * 10)
+                                + (o >> 5);
                             const uint32_t d0 = d[0];
                             const uint32_t d1 = d[1];
                             const uint32_t d2 = d[2];
@@ -696,31 +797,58 @@ struct RowTileQ {
         if (!__ballot((m[0] | m[1]) != 0)) {
             return true;
         }
-        const uint64_t word[2] = {ts + (rel[0] >> 6), ts + (rel[1] >> 6)};
+        const uint64_t word[2] = {ts + (rel6[0] >> 6), ts + (rel6[1] >> 6)};
+        const int den[2] = {static_cast<int>(rb[rs[0] + kRingDen]),
+                            static_cast<int>(rb[rs[1] + kRingDen])};
         // The outcome is wave-uniform; saying so keeps the caller's loop
         // (and its row offsets) in scalar registers.
-        return uniform(emit_words<2>(O, L.bound, m, word, b, request, lane) ? 1u : 0u) != 0;
+        return uniform(emit_words<2>(O, L.bound, m, word, den, request, lane) ? 1u : 0u) != 0;
     }
 
-    // One wave sweeps [ws, we); `from` is the resume cursor (W + u) or ws.
-    // Returns we when done, or the cursor of the oldest entry that did not
-    // fit the buffer.
-    __device__ static uint64_t sweep(uint64_t *pool, int b, uint64_t ts,
-                                     uint64_t from, uint64_t we, int wave,
-                                     const LineDesc &L, const OutDesc &O,
-                                     int request, int lane) {
-        const uint32_t *s_p = reinterpret_cast<const uint32_t *>(pool + Base::kSmall);
+    // Cursor of the oldest queued entry: its line, and W + u from its word
+    // offset r = (W - ts) + lane + 64 u (iterations start at multiples of
+    // kStride from ts, and lane + 64 u < kStride).
+    __device__ __forceinline__ static Cursor oldest(const uint64_t *pool, int wave, const Carry &q,
+                                                    uint64_t ts) {
+        wave_sync();
+        const uint32_t *qw = reinterpret_cast<const uint32_t *>(queue_masks(pool, wave) + kQueue);
+        const uint32_t r = uniform(qw[0]) & kRelMask;
+        return Cursor{q.first_line,
+                      ts + (r & ~static_cast<uint32_t>(kStride - 1)) + ((r & (kStride - 1)) >> 6)};
+    }
+
+    // Before run line next_line takes its ring slot (or, with `force`, at
+    // the end of the run): drains the queue when its oldest entry's slot is
+    // the one next_line takes, or when forced.  False (and `fail` set) when
+    // the survivors did not fit.
+    __device__ __forceinline__ static bool settle(const uint64_t *pool, uint64_t ts,
+                                                  const LineDesc &L, const OutDesc &O, int request,
+                                                  int lane, int wave, Carry &q, uint32_t next_line,
+                                                  bool force, Cursor &fail) {
+        if (q.queued == 0 || (!force && next_line - q.first_line < static_cast<uint32_t>(kRing))) {
+            return true;
+        }
+        if (!drain(pool, wave, q.queued, ts, L, O, request, lane)) {
+            fail = oldest(pool, wave, q, ts);
+            return false;
+        }
+        q.queued = 0;
+        return true;
+    }
+
+    // One wave sweeps [from, we) of run line `line` (ring slot `slot`);
+    // `from` is the resume cursor (W + u) or the wave's range start.  False
+    // (and `fail` set to the oldest uncommitted entry) when a drain's
+    // survivors did not fit the buffer; words left in the queue at the end
+    // stay there for a later drain.
+    __device__ __forceinline__ static bool sweep_line(uint64_t *pool, int /*b*/, uint64_t ts,
+                                                      uint64_t from, uint64_t we, int wave,
+                                                      const LineDesc &L, const OutDesc &O,
+                                                      int request, int lane, Carry &q,
+                                                      uint32_t line, int slot, Cursor &fail) {
+        const uint32_t *s_p = Base::small(pool, static_cast<int>(line & 1u));
         const uint32_t *s_base = s_p + 4 * kSelected;
         const uint32_t *s_magic = s_base + kSelected;
-        uint64_t *qm = pool + kQueueBase + wave * kQueueWords;
-        uint32_t *qw = reinterpret_cast<uint32_t *>(qm + kQueue);
-        uint32_t lp = 1, lmagic = 0, lcb = 0;
-        if (lane < kSelected - FAST) {
-            const uint32_t *s_c = s_p + kSelected;
-            lp = s_p[FAST + lane];
-            lmagic = s_magic[FAST + lane];
-            lcb = mod_small(s_c[FAST + lane] + 64u * s_base[FAST + lane], lp, lmagic);
-        }
         // Iterations start at ts + k kStride, so the sub-word is the cursor's
         // offset from the tile start modulo the stride.
         const uint32_t skip0 = static_cast<uint32_t>((from - ts) & (kStride - 1));
@@ -736,55 +864,38 @@ struct RowTileQ {
         uint32_t n_clean = clean_end > ws
             ? static_cast<uint32_t>((clean_end - ws) / kStride) : 0u;
         n_clean = n_clean < n_iter ? n_clean : n_iter;
-        Queue q{qm, qw, 0u};
-        const Late late{lp, lmagic, lcb};
+        const Line ln{line, static_cast<uint32_t>(slot) << kSlotShift};
         // Iteration it starts at word ws + it kStride, rel0 + it kStride from ts.
         const uint32_t rel0 = static_cast<uint32_t>(ws - ts);
         uint32_t it = 0;
         if (skip0 && n_iter) {
-            if (!step<true>(pool, ws, rel0, we, last_word, skip0, o, q, late, b, ts, L,
-                            O, request, lane)) {
-                return oldest(q, ts);
+            if (!step<true>(pool, ws, rel0, we, last_word, skip0, o, q, ln, wave, ts, L, O,
+                            request, lane)) {
+                fail = oldest(pool, wave, q, ts);
+                return false;
             }
             it = 1;
         }
         for (; it < n_clean; ++it) {
-            if (!step<false>(pool, 0, rel0 + it * kStride, we, last_word, 0u, o, q,
-                             late, b, ts, L, O, request, lane)) {
-                return oldest(q, ts);
+            if (!step<false>(pool, 0, rel0 + it * kStride, we, last_word, 0u, o, q, ln, wave,
+                             ts, L, O, request, lane)) {
+                fail = oldest(pool, wave, q, ts);
+                return false;
             }
         }
         for (; it < n_iter; ++it) {
             if (!step<true>(pool, ws + static_cast<uint64_t>(it) * kStride, rel0 + it * kStride,
-                            we, last_word, 0u, o, q, late, b, ts, L, O, request,
-                            lane)) {
-                return oldest(q, ts);
+                            we, last_word, 0u, o, q, ln, wave, ts, L, O, request, lane)) {
+                fail = oldest(pool, wave, q, ts);
+                return false;
             }
         }
-        if (q.queued && !drain(pool, q.qm, q.qw, late.p, late.magic, late.cb, q.queued, b, ts,
-                               L, O, request, lane)) {
-            return oldest(q, ts);
-        }
-        return we;
+        return true;
     }
 
-    struct Queue {
-        uint64_t *qm;
-        uint32_t *qw;
-        uint32_t queued;
-    };
-
-    // Resume cursor (W + u) of the oldest queued entry, from its word offset
-    // r = (W - ts) + lane + 64 u: iterations start at multiples of kStride
-    // from ts, and lane + 64 u < kStride.
-    __device__ static uint64_t oldest(const Queue &q, uint64_t ts) {
-        wave_sync();
-        const uint32_t r = uniform(q.qw[0]);
-        return ts + (r & ~static_cast<uint32_t>(kStride - 1)) + ((r & (kStride - 1)) >> 6);
-    }
-
-    struct Late {
-        uint32_t p, magic, cb;  // lane j: late prime FAST + j
+    struct Line {
+        uint32_t index;  // run line
+        uint32_t tag;    // ring slot, in entry-tag position
     };
 
     // One iteration at W: dense stage, (EDGE: masks for the range end, the
@@ -793,11 +904,11 @@ struct RowTileQ {
     // did not fit the buffer.
     // W is read only by the EDGE body; rel_w = W - ts.
     template <bool EDGE>
-    __device__ static bool step(uint64_t *pool, uint64_t W, uint32_t rel_w, uint64_t we,
-                                uint64_t last_word, uint32_t skip,
-                                typename Base::Offsets &o, Queue &q, const Late &late,
-                                int b, uint64_t ts, const LineDesc &L, const OutDesc &O,
-                                int request, int lane) {
+    __device__ __forceinline__ static bool step(uint64_t *pool, uint64_t W, uint32_t rel_w,
+                                                uint64_t we, uint64_t last_word, uint32_t skip,
+                                                typename Base::Offsets &o, Carry &q, const Line &ln,
+                                                int wave, uint64_t ts, const LineDesc &L,
+                                                const OutDesc &O, int request, int lane) {
         uint64_t m[U];
         Base::dense(pool, o, lane, m);
         if (EDGE) {
@@ -819,24 +930,28 @@ struct RowTileQ {
         if (!__ballot(any != 0)) {
             return true;
         }
-        const uint32_t rel = rel_w + lane;
+        uint64_t *qm = queue_masks(pool, wave);
+        uint32_t *qw = reinterpret_cast<uint32_t *>(qm + kQueue);
+        const uint32_t rel = (rel_w + lane) | ln.tag;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint64_t bal = __ballot(m[u] != 0);
             if (!bal) {
                 continue;
             }
+            if (q.queued == 0) {
+                q.first_line = ln.index;
+            }
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
                 static_cast<uint32_t>(bal >> 32),
                 __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bal), 0u));
             if (m[u]) {
-                q.qm[q.queued + rank] = m[u];
-                q.qw[q.queued + rank] = rel + 64u * u;
+                qm[q.queued + rank] = m[u];
+                qw[q.queued + rank] = rel + 64u * u;
             }
             q.queued += static_cast<uint32_t>(__popcll(bal));
             if (q.queued >= 64) {
-                if (!drain(pool, q.qm, q.qw, late.p, late.magic, late.cb, q.queued, b, ts, L,
-                           O, request, lane)) {
+                if (!drain(pool, wave, q.queued, ts, L, O, request, lane)) {
                     return false;
                 }
                 q.queued = 0;
@@ -880,56 +995,90 @@ __device__ __forceinline__ int select_rank(const uint32_t *cp, uint32_t b, int l
 // ---------------------------------------------------------------------------
 // Runs of lines.  A row tile covers nr consecutive denominators of one curve
 // (nr > 1 only when the tile is the whole line).  The curve's candidate
-// constants stay in LDS for the tile; per line, wave 0 selects the primes,
-// the workgroup builds the dense rows, and the residue strings of the next
-// line are loaded into registers while the current line is swept, so a line
-// costs no global-memory round trip after the first.
+// constants stay in LDS for the tile.  A line takes two phases, each ended
+// by a barrier:
+//   A(r)  every wave builds its share of line r's dense rows; wave 0
+//         selects the primes of line r + 1 and the last wave the candidate
+//         indices of line r + 2
+//   B(r)  every thread starts loading its string word of line r + 2 and
+//         stores that of line r + 1 (loaded during B(r - 1)) into place;
+//         every wave sweeps its share of line r
+// The per-line arrays are kept for two lines (by parity) so that a
+// selection can be written while the other line's arrays are read, and
+// nothing waits on global memory: a string word is loaded a whole line
+// before it is stored.
 // ---------------------------------------------------------------------------
 template <class Row>
 struct RowRun {
     using Base = typename Row::Base;
     static constexpr int kThreads = Row::kWaves * 64;
     // u32 arrays after the row tile's LDS: candidate p, magic and (-A) mod p
-    // [36 each], then the next line's selection: candidate index and b mod p
-    // [32 each].
+    // [36 each], then for two lines (by parity) the selection's candidate
+    // index and b mod p [32 each].
+    static constexpr int kFastRows = Base::kFast;
     static constexpr int kCand = Row::kLdsWords;
-    static constexpr int kLdsWords = kCand + (3 * kCandidates + 2 * kSelected + 1) / 2;
+    static constexpr int kLdsWords = kCand + (3 * kCandidates + 4 * kSelected + 1) / 2;
     static_assert(kSelected * kResidWords <= kThreads, "one string word per thread");
+    static_assert(Row::kRing >= 2, "line r + 1 takes its slot while line r is swept");
 
     // Cursor of a wave within a run: line index above bit 40, word below.
     static constexpr int kLineShift = 40;
     static constexpr uint64_t kPosMask = (1ULL << kLineShift) - 1;
 
-    __device__ static uint32_t *cand(uint64_t *pool) {
+    __device__ __forceinline__ static uint32_t *cand(uint64_t *pool) {
         return reinterpret_cast<uint32_t *>(pool + kCand);
     }
 
-    __device__ static void load_curve(uint64_t *pool, const LineDesc &L,
-                                      const int *primes, int tid) {
+    // Candidate index and b mod p of run line r's selection.
+    __device__ __forceinline__ static uint32_t *next_sel(uint64_t *pool, int r) {
+        return cand(pool) + 3 * kCandidates + (r & 1) * 2 * kSelected;
+    }
+
+    __device__ __forceinline__ static int slot_of(int r) {
+        return r % Row::kRing;
+    }
+
+    __device__ __forceinline__ static void load_curve(uint64_t *pool, const LineDesc &L,
+                                                      const int *primes, int tid) {
         load_candidates(cand(pool), L, primes, tid);
     }
 
-    // Wave 0: the selection of denominator b for the line chunk at ts into
-    // the row tile's arrays.
-    __device__ static void select(uint64_t *pool, uint32_t b, uint64_t ts, int lane) {
+    // Wave 0: the selection of run line r (denominator b) for the chunk at
+    // ts into the row tile's arrays of r's parity, and the line's constants
+    // into its ring slot.
+    __device__ __forceinline__ static void select(uint64_t *pool, uint32_t b, uint64_t ts, int r,
+                                                  int lane) {
         uint32_t *cp = cand(pool);
-        uint32_t *s_p = reinterpret_cast<uint32_t *>(pool + Base::kSmall);
+        uint32_t *s_p = Base::small(pool, r & 1);
         uint32_t p, magic, u;
         const int rank = select_rank(cp, b, lane, p, magic, u);
+        // Default selection: rank s is candidate 35 - s for every dense s.
+        const bool moved = lane >= kCandidates - kFastRows && lane < kCandidates
+                           && rank != kCandidates - 1 - lane;
+        const bool dflt = __ballot(moved) == 0;
+        if (lane == 0) {
+            s_p[Base::kDefaultFlag] = dflt ? 1u : 0u;
+        }
         if (rank >= 0) {
+            const uint32_t c = cp[2 * kCandidates + lane];
+            const uint32_t ts_mod = static_cast<uint32_t>(ts % p);
             s_p[rank] = p;
-            s_p[kSelected + rank] = cp[2 * kCandidates + lane];  // c
+            s_p[kSelected + rank] = c;
             s_p[2 * kSelected + rank] = u;
             s_p[3 * kSelected + rank] = static_cast<uint32_t>(lane);
-            s_p[4 * kSelected + rank] = static_cast<uint32_t>(ts % p);
+            s_p[4 * kSelected + rank] = ts_mod;
             s_p[5 * kSelected + rank] = magic;
+            // String offset of the tile's first word: c + 64 ts (mod p).
+            Row::note_line(pool, slot_of(r), rank, p, magic,
+                           mod_small(c + 64u * ts_mod, p, magic), b);
         }
     }
 
-    // Wave 0: candidate index and u of denominator b into the next arrays.
-    __device__ static void select_next(uint64_t *pool, uint32_t b, int lane) {
+    // One wave: candidate index and u of run line r (denominator b).
+    __device__ __forceinline__ static void select_next(uint64_t *pool, uint32_t b, int r,
+                                                       int lane) {
         uint32_t *cp = cand(pool);
-        uint32_t *nx = cp + 3 * kCandidates;
+        uint32_t *nx = next_sel(pool, r);
         uint32_t p, magic, u;
         const int rank = select_rank(cp, b, lane, p, magic, u);
         if (rank >= 0) {
@@ -938,59 +1087,87 @@ struct RowRun {
         }
     }
 
-    // One string word of the next line per thread (threads below 288).
-    __device__ static uint64_t fetch(uint64_t *pool, const LineDesc &L, int tid) {
+    // One string word of run line r per thread (threads below 288).
+    __device__ __forceinline__ static uint64_t fetch(uint64_t *pool, const LineDesc &L, int r,
+                                                     int tid) {
         if (tid >= kSelected * kResidWords) {
             return 0;
         }
-        const uint32_t *nx = cand(pool) + 3 * kCandidates;
+        const uint32_t *nx = next_sel(pool, r);
         const int s = tid / kResidWords;
         const int w = tid - s * kResidWords;
         return L.resid[(static_cast<size_t>(nx[s]) * kMaxPrime + nx[kSelected + s])
                        * kResidWords + w];
     }
 
-    __device__ static void store(uint64_t *pool, uint64_t v, int tid) {
+    __device__ __forceinline__ static void store(uint64_t *pool, uint64_t v, int r, int tid) {
         if (tid < kSelected * kResidWords) {
             const int s = tid / kResidWords;
             const int w = tid - s * kResidWords;
-            pool[Base::kCext + s * 10 + w] = v;
+            Row::string(pool, s, slot_of(r))[w] = v;
         }
     }
 
-    // Strings of denominator b into registers, before the first line.
-    __device__ static uint64_t begin(uint64_t *pool, const LineDesc &L, uint32_t b,
-                                     int tid, int lane, int wave) {
+    // Run start at line r0 of nr (first denominator t0): line r0's strings
+    // and selection in place, the candidate indices of r0 + 1 chosen; returns
+    // this thread's string word of line r0 + 1 (loading).  Ends with a
+    // barrier.
+    __device__ __forceinline__ static uint64_t begin(uint64_t *pool, const LineDesc &L, int t0,
+                                                     int r0, int nr, uint64_t ts, int tid, int lane,
+                                                     int wave) {
+        const uint32_t b0 = static_cast<uint32_t>(denominator(L, static_cast<long long>(t0) + r0));
         if (wave == 0) {
-            select_next(pool, b, lane);
+            select_next(pool, b0, r0, lane);
         }
         __syncthreads();
-        return fetch(pool, L, tid);
+        store(pool, fetch(pool, L, r0, tid), r0, tid);
+        if (wave == 0) {
+            select(pool, b0, ts, r0, lane);
+        }
+        if (wave == Row::kWaves - 1 && r0 + 1 < nr) {
+            select_next(pool,
+                        static_cast<uint32_t>(denominator(L, static_cast<long long>(t0) + r0 + 1)),
+                        r0 + 1, lane);
+        }
+        __syncthreads();
+        return r0 + 1 < nr ? fetch(pool, L, r0 + 1, tid) : 0;
     }
 
-    // Set-up of the line of denominator b (its strings in v); when `more`,
-    // starts loading the strings of denominator bn and returns them.
-    // `flag`, when given, is read between the barriers (no sweep runs then,
-    // so every wave sees the same value) into `flag_value`.
-    __device__ static uint64_t setup_line(uint64_t *pool, const LineDesc &L, uint32_t b,
-                                          uint64_t ts, bool more, uint32_t bn,
-                                          uint64_t v, int tid, int lane, int wave,
-                                          const int *flag, int &flag_value) {
-        __syncthreads();  // the previous line's sweep is over
+    // Phase A of run line r.  `flag`, when given, is read first (no sweep
+    // runs in this phase, so every wave sees the same value) into
+    // `flag_value`.  Ends with a barrier.
+    __device__ __forceinline__ static void build_line(uint64_t *pool, const LineDesc &L, int t0,
+                                                      int r, int nr, uint64_t ts,
+                                                      const typename Base::Plan &plan, int tid,
+                                                      int lane, int wave, const int *flag,
+                                                      int &flag_value) {
         flag_value = flag ? static_cast<int>(uniform(static_cast<uint32_t>(*flag))) : 0;
-        store(pool, v, tid);
-        // Two waves select in parallel: this line's primes (wave 0) and the
-        // next line's candidate indices (the last wave).
-        if (wave == 0) {
-            select(pool, b, ts, lane);
+        if (Base::kPlanned && uniform(Base::small(pool, r & 1)[Base::kDefaultFlag])) {
+            Base::build_planned(pool, plan, tid, lane, wave);
+        } else {
+            Base::build_rows(pool, tid, r & 1);
         }
-        if (wave == Row::kWaves - 1 && more) {
-            select_next(pool, bn, lane);
+        if (wave == 0 && r + 1 < nr) {
+            select(pool,
+                   static_cast<uint32_t>(denominator(L, static_cast<long long>(t0) + r + 1)),
+                   ts, r + 1, lane);
+        }
+        if (wave == Row::kWaves - 1 && r + 2 < nr) {
+            select_next(pool,
+                        static_cast<uint32_t>(denominator(L, static_cast<long long>(t0) + r + 2)),
+                        r + 2, lane);
         }
         __syncthreads();
-        Base::build_rows(pool, tid);
-        const uint64_t nv = more ? fetch(pool, L, tid) : 0;
-        __syncthreads();
+    }
+
+    // Start of phase B of run line r: v (this thread's word of line r + 1)
+    // into place; returns its word of line r + 2 (loading).
+    __device__ __forceinline__ static uint64_t advance(uint64_t *pool, const LineDesc &L, int r,
+                                                       int nr, uint64_t v, int tid) {
+        const uint64_t nv = r + 2 < nr ? fetch(pool, L, r + 2, tid) : 0;
+        if (r + 1 < nr) {
+            store(pool, v, r + 1, tid);
+        }
         return nv;
     }
 };
@@ -1009,8 +1186,8 @@ struct DirectTile {
     static constexpr int kPrm = NB * kSelected * 10;
     static constexpr int kLdsWords = kPrm + NB * kSelected;
 
-    __device__ static void wave_range(uint32_t total, int wave, uint32_t &qs,
-                                      uint32_t &qe) {
+    __device__ __forceinline__ static void wave_range(uint32_t total, int wave, uint32_t &qs,
+                                                      uint32_t &qe) {
         const uint32_t per = ((total + 64u * WAVES - 1) / (64u * WAVES)) * 64u;
         qs = per * wave;
         qe = qs + per < total ? qs + per : total;
@@ -1021,9 +1198,8 @@ struct DirectTile {
 
     // Prime selection per denominator (src/sieve_plan.cpp:75-86), one wave
     // per denominator; the lane that ranks a prime stages its string.
-    __device__ static void setup(uint64_t *pool, int t0, int nb,
-                                 const LineDesc &L, const int *primes,
-                                 int lane, int wave) {
+    __device__ __forceinline__ static void setup(uint64_t *pool, int t0, int nb, const LineDesc &L,
+                                                 const int *primes, int lane, int wave) {
         uint32_t *prm = reinterpret_cast<uint32_t *>(pool + kPrm);
         for (int bl = wave; bl < nb; bl += WAVES) {
             const uint32_t b = static_cast<uint32_t>(
@@ -1053,9 +1229,9 @@ struct DirectTile {
 
     // One wave sweeps pairs [qs, qe) (pair q = line q / words, word q mod
     // words).  Returns qe when done, or the iteration that did not fit.
-    __device__ static uint32_t sweep(const uint64_t *pool, int t0, uint32_t qs,
-                                     uint32_t qe, const LineDesc &L,
-                                     const OutDesc &O, int request, int lane) {
+    __device__ __forceinline__ static uint32_t sweep(const uint64_t *pool, int t0, uint32_t qs,
+                                                     uint32_t qe, const LineDesc &L,
+                                                     const OutDesc &O, int request, int lane) {
         const uint32_t *prm = reinterpret_cast<const uint32_t *>(pool + kPrm);
         const uint32_t words = static_cast<uint32_t>(L.words);
         uint32_t Q = qs;
@@ -1115,19 +1291,19 @@ struct DirectRun {
     static constexpr int kBlockShift = 40;
     static constexpr uint64_t kPosMask = (1ULL << kBlockShift) - 1;
 
-    __device__ static uint32_t *cand(uint64_t *pool) {
+    __device__ __forceinline__ static uint32_t *cand(uint64_t *pool) {
         return reinterpret_cast<uint32_t *>(pool + kCand);
     }
 
-    __device__ static void load_curve(uint64_t *pool, const LineDesc &L,
-                                      const int *primes, int tid) {
+    __device__ __forceinline__ static void load_curve(uint64_t *pool, const LineDesc &L,
+                                                      const int *primes, int tid) {
         load_candidates(cand(pool), L, primes, tid);
     }
 
     // Set-up of block [t0, t0 + nb); every thread calls it after the
     // previous block's sweep.  Ends with a barrier.
-    __device__ static void setup(uint64_t *pool, const LineDesc &L, int t0, int nb,
-                                 int tid, int lane, int wave) {
+    __device__ __forceinline__ static void setup(uint64_t *pool, const LineDesc &L, int t0, int nb,
+                                                 int tid, int lane, int wave) {
         const uint32_t *cp = cand(pool);
         uint32_t *prm = reinterpret_cast<uint32_t *>(pool + Tile::kPrm);
         uint32_t *nx = reinterpret_cast<uint32_t *>(pool + kNext);
@@ -1303,25 +1479,38 @@ __global__ __launch_bounds__(WAVES * 64, 4) void sieve_kernel(
     if (r0 >= nr) {
         return;
     }
-    uint64_t v = Run::begin(pool, L, static_cast<uint32_t>(denominator(L, t0 + r0)), tid,
-                            lane, wave);
+    const typename Tile::Base::Plan plan = Tile::Base::make_plan(Run::cand(pool), tid, lane, wave);
+    uint64_t v = Run::begin(pool, L, t0, r0, nr, ts, tid, lane, wave);
     bool failed = false;
+    typename Tile::Carry carry;
+    Cursor fail{0u, 0};
     for (int r = r0; r < nr; ++r) {
-        const bool more = r + 1 < nr;
-        const int b = denominator(L, static_cast<long long>(t0) + r);
-        const int bn = more ? denominator(L, static_cast<long long>(t0) + r + 1) : 0;
         int unused;
-        v = Run::setup_line(pool, L, static_cast<uint32_t>(b), ts, more,
-                            static_cast<uint32_t>(bn), v, tid, lane, wave, nullptr, unused);
+        Run::build_line(pool, L, t0, r, nr, ts, plan, tid, lane, wave, nullptr, unused);
+        v = Run::advance(pool, L, r, nr, v, tid);
         if (!failed && static_cast<uint32_t>(r) >= cr && ws < we) {
             const uint64_t from = static_cast<uint32_t>(r) == cr ? cpos : ws;
-            const uint64_t stop = Tile::sweep(pool, b, ts, from, we, wave, L, P.out, 0, lane);
-            if (stop != we) {
+            if (!Tile::sweep_line(pool, denominator(L, static_cast<long long>(t0) + r), ts, from,
+                                  we, wave, L, P.out, 0, lane, carry, static_cast<uint32_t>(r),
+                                  Run::slot_of(r), fail)) {
                 failed = true;
-                cr = static_cast<uint32_t>(r);
-                cpos = stop;
             }
         }
+        // Line r + 2 takes the ring slot of line r + 2 - kRing in the next
+        // line's phases: queued words of that line are drained first.
+        if (!failed && !Tile::settle(pool, ts, L, P.out, 0, lane, wave, carry,
+                                     static_cast<uint32_t>(r + 2), false, fail)) {
+            failed = true;
+        }
+        __syncthreads();
+    }
+    if (!failed && !Tile::settle(pool, ts, L, P.out, 0, lane, wave, carry,
+                                 static_cast<uint32_t>(nr), true, fail)) {
+        failed = true;
+    }
+    if (failed) {
+        cr = fail.line;
+        cpos = fail.pos;
     }
     if (lane == 0) {
         *cursor = failed ? (static_cast<uint64_t>(cr) << Run::kLineShift) | cpos
@@ -1502,26 +1691,50 @@ __global__ __launch_bounds__(WAVES * 64, 4) void multi_kernel(
                 Row::wave_range(ts, te, wave, ws, we);
                 Run::load_curve(pool, L, M.primes, tid);
                 __syncthreads();
-                uint64_t v = Run::begin(pool, L, static_cast<uint32_t>(denominator(L, t0)), tid,
-                                        lane, wave);
+                const typename Row::Base::Plan plan =
+                    Row::Base::make_plan(Run::cand(pool), tid, lane, wave);
+                uint64_t v = Run::begin(pool, L, t0, 0, nr, ts, tid, lane, wave);
+                typename Row::Carry carry;
+                Cursor unused_cursor{0u, 0};
+                bool stopped = false;
                 for (int r = 0; r < nr; ++r) {
-                    const bool more = r + 1 < nr;
-                    const int b = denominator(L, static_cast<long long>(t0) + r);
-                    const int bn = more ? denominator(L, static_cast<long long>(t0) + r + 1) : 0;
                     int stop_run;
-                    v = Run::setup_line(pool, L, static_cast<uint32_t>(b), ts, more,
-                                        static_cast<uint32_t>(bn), v, tid, lane, wave, &s_fail,
-                                        stop_run);
+                    Run::build_line(pool, L, t0, r, nr, ts, plan, tid, lane, wave, &s_fail,
+                                    stop_run);
                     if (stop_run) {
+                        stopped = true;
                         break;
                     }
-                    if (ws < we
-                        && Row::sweep(pool, b, ts, ws, we, wave, L, M.out, C.request, lane) != we) {
+                    v = Run::advance(pool, L, r, nr, v, tid);
+                    if (!failed && ws < we
+                        && !Row::sweep_line(pool, denominator(L, static_cast<long long>(t0) + r),
+                                            ts, ws, we, wave, L, M.out, C.request, lane, carry,
+                                            static_cast<uint32_t>(r), Run::slot_of(r),
+                                            unused_cursor)) {
                         failed = true;
                         if (lane == 0) {
                             s_fail = 1;
                         }
                     }
+                    // Queued words of the line whose ring slot line r + 2
+                    // takes are drained first; s_fail, set before the
+                    // barrier, is read by every wave at the next phase A.
+                    if (!failed && !Row::settle(pool, ts, L, M.out, C.request, lane, wave, carry,
+                                                static_cast<uint32_t>(r + 2), false,
+                                                unused_cursor)) {
+                        failed = true;
+                        if (lane == 0) {
+                            s_fail = 1;
+                        }
+                    }
+                    __syncthreads();
+                }
+                // A stopped run's curve is marked failed and searched again
+                // on the plain path, so its queued words are dropped.
+                if (!failed && !stopped
+                    && !Row::settle(pool, ts, L, M.out, C.request, lane, wave, carry,
+                                    static_cast<uint32_t>(nr), true, unused_cursor)) {
+                    failed = true;
                 }
             }
             if (failed && lane == 0) {
@@ -1556,11 +1769,8 @@ hipError_t multi_occupancy(int *blocks) {
                                                         WAVES * 64, 0);
 }
 
-// "u2_f14_w8": RowTile with U = 2, FAST = 14, 8 waves; "q_u2_f12_w8": the
-// same shape with the compaction-queue late stage.
-#define RPS_VARIANT(U, F, W)                                               \
-    {"u" #U "_f" #F "_w" #W, U, F, W, 17, 0, launch_variant<RowTile<U, F, W>, W>, \
-     launch_multi_variant<RowTile<U, F, W>, W>, multi_occupancy<RowTile<U, F, W>, W>}
+// "q_u2_f12_w8": RowTileQ with U = 2, FAST = 12, 8 waves; "w_u2_f14_w16":
+// the wide-row tile with FAST = 14, 16 waves.
 #define RPS_VARIANT_Q(U, F, W)                                             \
     {"q_u" #U "_f" #F "_w" #W, U, F, W, 17, 0,                            \
      launch_variant<RowTileQ<U, F, W>, W>,                                 \
@@ -1578,12 +1788,10 @@ const VariantInfo kVariants[] = {
     RPS_VARIANT_Q(2, 10, 8),
     RPS_VARIANT_Q(1, 12, 8),
     RPS_VARIANT_Q(2, 12, 16),
-    RPS_VARIANT(2, 14, 8),
     RPS_VARIANT_W(14, 16),
     RPS_VARIANT_W(13, 16),
 };
 
-#undef RPS_VARIANT
 #undef RPS_VARIANT_Q
 #undef RPS_VARIANT_W
 

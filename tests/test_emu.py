@@ -21,6 +21,7 @@ import numpy as np
 import pytest
 
 import curves
+import oracle
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EMU_DIR = os.path.join(ROOT, "tests", "emu", "_build")
@@ -43,6 +44,11 @@ def emu(pkg):
     ctx.close()
 
 
+def variants(ctx):
+    """Variant names of the library behind `ctx` (the emulated one here)."""
+    return [ctx.lib.rps_variant_name(i).decode() for i in range(ctx.lib.rps_variant_count())]
+
+
 def defaults(ctx):
     for name, value in (("variant", -1), ("survivor_capacity", 1 << 24), ("batch_size", 65536),
                         ("min_tiles", 2048), ("direct_words", 768), ("primitive_only", 0)):
@@ -56,28 +62,67 @@ def same(res, ref):
             and np.array_equal(res.denominators, den))
 
 
-def test_every_variant_matches_oracle(emu, pkg, oracle_lib):
+def test_every_variant_matches_oracle(emu, oracle_lib):
     # One long-line case per compiled variant: lines of 1,251 words in runs
     # (min_tiles 1), denominators divisible by candidate primes included.
     coeffs, _ = curves.partially_dense(random.Random(5), 7, 6)
     ref = oracle_lib.sieve(coeffs, 40000, 437, 446, method="words")
-    for v, name in enumerate(pkg.variants()):
+    for v, name in enumerate(variants(emu)):
         defaults(emu).set_option("variant", v)
         emu.set_option("min_tiles", 1)
         assert same(emu.sieve(coeffs, 40000, 437, 446), ref), name
 
 
-def test_resume_exactly_once(emu, pkg, oracle_lib):
+def test_resume_exactly_once(emu, oracle_lib):
     # About a quarter of all sites survive: a 16,384-entry buffer overflows
     # in every launch and the waves resume at their cursors.
     coeffs, _ = curves.partially_dense(random.Random(78), 2, 4)
     ref = oracle_lib.sieve(coeffs, 20000, 3, 6, method="words")
-    for name in ("q_u1_f12_w8", "w_u2_f14_w16"):
-        defaults(emu).set_option("variant", pkg.variants().index(name))
+    for name in ("q_u1_f12_w8", "q_u2_f12_w8", "w_u2_f14_w16"):
+        defaults(emu).set_option("variant", variants(emu).index(name))
         emu.set_option("survivor_capacity", 16384)
         res = emu.sieve(coeffs, 20000, 3, 6)
         assert res.stats["launches"] > 2
         assert same(res, ref), name
+
+
+def carry_curve():
+    # Nine of the twelve dense primes live and every late prime dead: about
+    # an eighth of the words pass the dense rows, so each wave queues a few
+    # dozen words per 2,000-word line and its queue spans several lines,
+    # while an eighth of a survivor per word fills a 16,384-entry buffer
+    # every ~65 lines.
+    dense = sorted(oracle.candidate_primes(), reverse=True)[:12]
+    return curves.with_live_primes(random.Random(31), dense[:9], 6)
+
+
+def test_queue_carried_across_lines_resumes_exactly_once(emu, oracle_lib):
+    coeffs = carry_curve()
+    ref = oracle_lib.sieve(coeffs, 64000, 1, 160, method="words")
+    for name in ("q_u1_f12_w8", "q_u2_f12_w8", "w_u2_f14_w16"):
+        defaults(emu).set_option("variant", variants(emu).index(name))
+        emu.set_option("min_tiles", 1)
+        emu.set_option("survivor_capacity", 16384)
+        res = emu.sieve(coeffs, 64000, 1, 160)
+        assert res.stats["launches"] > 2, name
+        assert same(res, ref), name
+
+
+def test_queue_carried_across_lines_many_path(emu, oracle_lib):
+    # The same run through the persistent launch: a full buffer marks the
+    # curve failed mid-run and the host searches it again on the plain path.
+    reqs = [{"coefficients": carry_curve(), "bound": 64000, "first": 1, "last": 160},
+            {"coefficients": ["1", "0", "3", "0", "0", "-7", "2"], "bound": 64000,
+             "first": 1, "last": 40}]
+    defaults(emu).set_option("variant", variants(emu).index("q_u1_f12_w8"))
+    emu.set_option("min_tiles", 1)
+    emu.set_option("survivor_capacity", 16384)
+    results, stats = emu.sieve_many(reqs)
+    assert stats["requeued"] >= 1
+    for r, (num, den) in zip(reqs, results):
+        count, digest, on, od = oracle_lib.sieve(r["coefficients"], r["bound"], r["first"],
+                                                 r["last"], method="words")
+        assert np.array_equal(num, on) and np.array_equal(den, od)
 
 
 def test_short_lines_and_square_mode(emu, oracle_lib):
