@@ -123,9 +123,13 @@ struct TimedOut {
     std::string message;
 };
 
+// The first millisecond is spent polling (a short launch is seen at once),
+// later polls are 20 us apart.
 void wait_event(hipEvent_t event, long long timeout_ms, const std::string &what) {
-    const auto deadline = Clock::now() + std::chrono::milliseconds(timeout_ms);
-    for (unsigned spins = 0;; ++spins) {
+    const auto start = Clock::now();
+    const auto deadline = start + std::chrono::milliseconds(timeout_ms);
+    const auto spin_until = start + std::chrono::milliseconds(1);
+    for (;;) {
         const hipError_t e = hipEventQuery(event);
         if (e == hipSuccess) {
             return;
@@ -133,11 +137,14 @@ void wait_event(hipEvent_t event, long long timeout_ms, const std::string &what)
         if (e != hipErrorNotReady) {
             throw HipError{e, "hipEventQuery"};
         }
-        if (Clock::now() > deadline) {
+        const auto now = Clock::now();
+        if (now > deadline) {
             throw TimedOut{what + " did not finish within " + std::to_string(timeout_ms) + " ms"};
         }
-        if (spins > 64) {
-            std::this_thread::sleep_for(std::chrono::microseconds(50));
+        if (now < spin_until) {
+            std::this_thread::yield();
+        } else {
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
         }
     }
 }
@@ -186,6 +193,70 @@ struct Grow {
     }
 };
 
+// Pinned host buffer that only grows.
+template <typename T>
+struct PinnedGrow {
+    T *ptr = nullptr;
+    size_t size = 0;
+
+    void ensure(size_t n) {
+        if (n <= size) {
+            return;
+        }
+        if (ptr) {
+            RPS_HIP(hipHostFree(ptr));
+            ptr = nullptr;
+            size = 0;
+        }
+        RPS_HIP(hipHostMalloc(reinterpret_cast<void **>(&ptr), n * sizeof(T), 0));
+        size = n;
+    }
+
+    void release() {
+        if (ptr) {
+            (void)hipHostFree(ptr);
+        }
+        ptr = nullptr;
+        size = 0;
+    }
+};
+
+// Byte offsets of the sections of a many-curve call's plan, each aligned to
+// 256 bytes, so that the whole plan goes to the device in one copy.
+struct PlanLayout {
+    size_t coef = 0, slot = 0, negmod = 0, curves = 0, tile_start = 0, bucket = 0, bytes = 0;
+
+    PlanLayout(size_t coef_ints, size_t n_build, size_t n, size_t n_buckets) {
+        auto take = [this](size_t b) {
+            const size_t at = bytes;
+            bytes += (std::max<size_t>(b, 1) + 255) / 256 * 256;
+            return at;
+        };
+        coef = take(coef_ints * sizeof(int));
+        slot = take(n_build * sizeof(int));
+        negmod = take(n * rps::kCandidates * sizeof(int));
+        curves = take(n * sizeof(rps::CurveDesc));
+        tile_start = take((n + 1) * sizeof(unsigned long long));
+        bucket = take(n_buckets * sizeof(int));
+    }
+};
+
+// Control block of a many-curve call on the device: the survivor state, the
+// tile counter and the per-curve failure flags, read back in one copy after
+// every launch.
+struct ControlLayout {
+    static constexpr size_t kState = 0;
+    static constexpr size_t kHead = 32;
+    static constexpr size_t kFailed = 64;
+    static_assert(sizeof(rps::DeviceState) <= kHead, "state fits its section");
+    static size_t bytes(size_t n) { return kFailed + n * sizeof(int); }
+};
+
+// Survivors of a many-curve launch copied back with its control block
+// before their count is known: a launch with at most this many needs no
+// second copy.
+constexpr size_t kSpeculativeSurvivors = 4096;
+
 }  // namespace
 
 struct rps_ctx {
@@ -227,24 +298,21 @@ struct rps_ctx {
     uint64_t last_launches = 0;
     double last_sites = 0.0;
     // Many-curve launches.
-    Grow<int> m_coef;
     Grow<unsigned long long> m_chi;
     // Residue tables of recent curves stay on the device between calls of
     // rps_sieve_many; the least recently used are replaced.
     Grow<unsigned long long> t_resid;  // [t_cap][36][512][9]
-    Grow<int> t_slot;                  // table slot of each curve being built
     int t_cap = 0;
     long long table_slots = 256;
     std::map<std::vector<std::string>, int> t_index;
     std::vector<std::vector<std::string>> t_key;  // per slot
     std::vector<uint64_t> t_used;                 // per slot: last call
     uint64_t t_call = 0;
-    Grow<int> m_negmod;
-    Grow<rps::CurveDesc> m_curves;
-    Grow<unsigned long long> m_tile_start;
-    Grow<int> m_bucket;
-    Grow<int> m_failed;
-    Grow<unsigned long long> m_head;
+    Grow<unsigned char> m_plan;         // PlanLayout: coefficient residues, table
+                                        // slots, (-A) mod p, curves, tiles, buckets
+    PinnedGrow<unsigned char> h_plan;   // its host copy
+    Grow<unsigned char> m_ctrl;         // ControlLayout
+    PinnedGrow<unsigned char> h_ctrl;
     Grow<int> m_req;
 };
 
@@ -592,16 +660,12 @@ void rps_close(rps_ctx *ctx) {
     if (ctx->h_num) (void)hipHostFree(ctx->h_num);
     if (ctx->h_den) (void)hipHostFree(ctx->h_den);
     if (ctx->h_req) (void)hipHostFree(ctx->h_req);
-    ctx->m_coef.release();
     ctx->m_chi.release();
     ctx->t_resid.release();
-    ctx->t_slot.release();
-    ctx->m_negmod.release();
-    ctx->m_curves.release();
-    ctx->m_tile_start.release();
-    ctx->m_bucket.release();
-    ctx->m_failed.release();
-    ctx->m_head.release();
+    ctx->m_plan.release();
+    ctx->h_plan.release();
+    ctx->m_ctrl.release();
+    ctx->h_ctrl.release();
     ctx->m_req.release();
     if (ctx->basis_start) (void)hipEventDestroy(ctx->basis_start);
     if (ctx->basis_end) (void)hipEventDestroy(ctx->basis_end);
@@ -1269,36 +1333,15 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
             }
         } guard{ctx};
 
-        ctx->m_coef.ensure(std::max<size_t>(1, static_cast<size_t>(n_build) * rps::kCandidates * n_coef));
         ctx->m_chi.ensure(std::max<size_t>(1, static_cast<size_t>(n_build) * rps::kCandidates * rps::kChiWords));
-        ctx->t_slot.ensure(std::max(1, n_build));
-        ctx->m_negmod.ensure(static_cast<size_t>(n) * rps::kCandidates);
-        ctx->m_curves.ensure(n);
-        ctx->m_tile_start.ensure(n + 1);
-        ctx->m_failed.ensure(n);
-        ctx->m_head.ensure(1);
         ctx->m_req.ensure(ctx->capacity);
         ensure_capacity(ctx);
-        std::vector<int> coef(static_cast<size_t>(n_build) * rps::kCandidates * n_coef);
-        for (int j = 0; j < n_build; ++j) {
-            coefficient_residues(&requests[order[build_k[j]]], ctx->primes, n_coef,
-                                 &coef[static_cast<size_t>(j) * rps::kCandidates * n_coef]);
-        }
-        std::vector<int> negmod(static_cast<size_t>(n) * rps::kCandidates);
         std::vector<rps::CurveDesc> curves(n);
         std::vector<unsigned long long> tile_start(n + 1, 0);
         for (int k = 0; k < n; ++k) {
-            const int r = order[k];
-            for (int i = 0; i < rps::kCandidates; ++i) {
-                const long long p = ctx->primes[i];
-                negmod[static_cast<size_t>(k) * rps::kCandidates + i] =
-                    static_cast<int>(((-requests[r].numerator_bound) % p + p) % p);
-            }
             rps::CurveDesc &C = curves[k];
-            C.line = lines[r];
-            C.line.resid = ctx->t_resid.ptr + static_cast<size_t>(slot_of[k]) * rps::kResidTableWords;
-            C.line.neg_bound_mod = ctx->m_negmod.ptr + static_cast<size_t>(k) * rps::kCandidates;
-            C.request = r;
+            C.line = lines[order[k]];
+            C.request = order[k];
             C.tile_start = tile_start[k];
             if (__builtin_add_overflow(tile_start[k], tile_count(C.line), &tile_start[k + 1])
                 || tile_start[k + 1] > (1ULL << 62)) {
@@ -1315,7 +1358,40 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
         }
         const size_t n_buckets = static_cast<size_t>(
             ((total_tiles + (1ULL << bucket_shift) - 1) >> bucket_shift) + 1);
-        std::vector<int> bucket(n_buckets);
+
+        // The plan goes to the device in one copy from pinned memory.
+        const size_t coef_ints = static_cast<size_t>(n_build) * rps::kCandidates * n_coef;
+        const PlanLayout plan(coef_ints, static_cast<size_t>(n_build), static_cast<size_t>(n),
+                              n_buckets);
+        ctx->m_plan.ensure(plan.bytes);
+        ctx->h_plan.ensure(plan.bytes);
+        ctx->m_ctrl.ensure(ControlLayout::bytes(static_cast<size_t>(n)));
+        ctx->h_ctrl.ensure(ControlLayout::bytes(static_cast<size_t>(n)));
+        unsigned char *hp = ctx->h_plan.ptr;
+        unsigned char *dp = ctx->m_plan.ptr;
+        int *h_coef = reinterpret_cast<int *>(hp + plan.coef);
+        for (int j = 0; j < n_build; ++j) {
+            coefficient_residues(&requests[order[build_k[j]]], ctx->primes, n_coef,
+                                 h_coef + static_cast<size_t>(j) * rps::kCandidates * n_coef);
+        }
+        std::copy(build_slot.begin(), build_slot.end(), reinterpret_cast<int *>(hp + plan.slot));
+        int *h_negmod = reinterpret_cast<int *>(hp + plan.negmod);
+        int *d_negmod = reinterpret_cast<int *>(dp + plan.negmod);
+        for (int k = 0; k < n; ++k) {
+            const int r = order[k];
+            for (int i = 0; i < rps::kCandidates; ++i) {
+                const long long p = ctx->primes[i];
+                h_negmod[static_cast<size_t>(k) * rps::kCandidates + i] =
+                    static_cast<int>(((-requests[r].numerator_bound) % p + p) % p);
+            }
+            curves[k].line.resid = ctx->t_resid.ptr
+                + static_cast<size_t>(slot_of[k]) * rps::kResidTableWords;
+            curves[k].line.neg_bound_mod = d_negmod + static_cast<size_t>(k) * rps::kCandidates;
+        }
+        std::memcpy(hp + plan.curves, curves.data(), n * sizeof(rps::CurveDesc));
+        std::memcpy(hp + plan.tile_start, tile_start.data(),
+                    (n + 1) * sizeof(unsigned long long));
+        int *h_bucket = reinterpret_cast<int *>(hp + plan.bucket);
         for (size_t g = 0, k = 0; g < n_buckets; ++g) {
             const unsigned long long first = std::min<unsigned long long>(
                 static_cast<unsigned long long>(g) << bucket_shift,
@@ -1323,61 +1399,48 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
             while (k + 1 < static_cast<size_t>(n) && tile_start[k + 1] <= first) {
                 ++k;
             }
-            bucket[g] = static_cast<int>(k);
+            h_bucket[g] = static_cast<int>(k);
         }
-        ctx->m_bucket.ensure(n_buckets);
         const auto t_workspace = Clock::now();
         st.plan_ms = ms_between(t_plan, t_workspace);
-        if (n_build) {
-            RPS_HIP(hipMemcpy(ctx->m_coef.ptr, coef.data(), coef.size() * sizeof(int),
-                              hipMemcpyHostToDevice));
-            RPS_HIP(hipMemcpy(ctx->t_slot.ptr, build_slot.data(), n_build * sizeof(int),
-                              hipMemcpyHostToDevice));
-        }
-        RPS_HIP(hipMemcpy(ctx->m_negmod.ptr, negmod.data(), negmod.size() * sizeof(int),
-                          hipMemcpyHostToDevice));
-        RPS_HIP(hipMemcpy(ctx->m_curves.ptr, curves.data(), n * sizeof(rps::CurveDesc),
-                          hipMemcpyHostToDevice));
-        RPS_HIP(hipMemcpy(ctx->m_tile_start.ptr, tile_start.data(),
-                          (n + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice));
-        RPS_HIP(hipMemcpy(ctx->m_bucket.ptr, bucket.data(), n_buckets * sizeof(int),
-                          hipMemcpyHostToDevice));
-        RPS_HIP(hipMemsetAsync(ctx->m_failed.ptr, 0, n * sizeof(int), ctx->compute));
-        RPS_HIP(hipMemsetAsync(ctx->m_head.ptr, 0, sizeof(unsigned long long), ctx->compute));
+        RPS_HIP(hipMemcpyAsync(dp, hp, plan.bytes, hipMemcpyHostToDevice, ctx->compute));
+        unsigned char *ctrl = ctx->m_ctrl.ptr;
+        rps::DeviceState *d_state = reinterpret_cast<rps::DeviceState *>(ctrl + ControlLayout::kState);
+        auto *d_head = reinterpret_cast<unsigned long long *>(ctrl + ControlLayout::kHead);
+        int *d_failed = reinterpret_cast<int *>(ctrl + ControlLayout::kFailed);
+        RPS_HIP(rps::launch_reset_many(d_state, d_head, d_failed, n, ctx->compute));
         const auto t_basis = Clock::now();
         st.workspace_ms = ms_between(t_workspace, t_basis);
         RPS_HIP(hipEventRecord(ctx->basis_start, ctx->compute));
         if (n_build) {
-            RPS_HIP(rps::launch_chi(ctx->m_coef.ptr, n_coef, n_build, ctx->d_primes,
-                                    ctx->m_chi.ptr, ctx->compute));
-            RPS_HIP(rps::launch_resid(ctx->d_primes, ctx->m_chi.ptr, n_build, ctx->t_slot.ptr,
+            RPS_HIP(rps::launch_chi(reinterpret_cast<const int *>(dp + plan.coef), n_coef, n_build,
+                                    ctx->d_primes, ctx->m_chi.ptr, ctx->compute));
+            RPS_HIP(rps::launch_resid(ctx->d_primes, ctx->m_chi.ptr, n_build,
+                                      reinterpret_cast<const int *>(dp + plan.slot),
                                       ctx->t_resid.ptr, ctx->compute));
         }
         RPS_HIP(hipEventRecord(ctx->basis_end, ctx->compute));
-        wait_event(ctx->basis_end, ctx->launch_timeout_ms, "residue-table build");
-        guard.built = true;
-        float bms = 0.0f;
-        RPS_HIP(hipEventElapsedTime(&bms, ctx->basis_start, ctx->basis_end));
-        st.basis_ms = bms;
         const auto t_sieve = Clock::now();
         st.basis_wall_ms = ms_between(t_basis, t_sieve);
 
-        // Persistent launches until every tile was handed out.  Survivors
-        // are kept per request on the host until the end, because a request
-        // whose reservation fails is searched again on the plain path.
+        // Persistent launches until every tile was handed out.  Each launch's
+        // control block and first survivors come back in the same stream,
+        // so a launch costs one wait.  Survivors are kept per request on the
+        // host until the end, because a request whose reservation fails is
+        // searched again on the plain path.
         Slot &s = ctx->slots[0];
         rps::MultiParams M{};
-        M.curves = ctx->m_curves.ptr;
-        M.tile_start = ctx->m_tile_start.ptr;
-        M.bucket = ctx->m_bucket.ptr;
+        M.curves = reinterpret_cast<const rps::CurveDesc *>(dp + plan.curves);
+        M.tile_start = reinterpret_cast<const unsigned long long *>(dp + plan.tile_start);
+        M.bucket = reinterpret_cast<const int *>(dp + plan.bucket);
         M.bucket_shift = bucket_shift;
-        M.curve_failed = ctx->m_failed.ptr;
-        M.head = ctx->m_head.ptr;
+        M.curve_failed = d_failed;
+        M.head = d_head;
         M.total_tiles = total_tiles;
         M.out.num = s.d_num;
         M.out.den = s.d_den;
         M.out.req = ctx->m_req.ptr;
-        M.out.state = s.d_state;
+        M.out.state = d_state;
         M.out.capacity = ctx->capacity;
         M.out.primitive = ctx->primitive;
         M.primes = ctx->d_primes;
@@ -1387,34 +1450,60 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
         RPS_HIP(var->multi_blocks_per_cu(&per_cu));
         const unsigned grid = static_cast<unsigned>(ctx->cu_count)
             * static_cast<unsigned>(std::max(1, per_cu));
+        const size_t spec = std::min({kSpeculativeSurvivors, ctx->stage_capacity, ctx->capacity});
+        const size_t ctrl_bytes = ControlLayout::bytes(static_cast<size_t>(n));
         std::vector<std::vector<long long>> kept_num(n);
         std::vector<std::vector<int>> kept_den(n);
+        auto keep = [&](size_t k) {
+            for (size_t i = 0; i < k; ++i) {
+                const int r = ctx->h_req[i];
+                if (r < 0 || r >= n) {
+                    throw ApiError{RPS_EINTERNAL, "survivor with an invalid request index"};
+                }
+                kept_num[r].push_back(ctx->h_num[i]);
+                kept_den[r].push_back(ctx->h_den[i]);
+            }
+        };
         unsigned long long head = 0;
         while (head < total_tiles) {
-            RPS_HIP(rps::launch_reset(s.d_state, ctx->compute));
+            if (st.launches) {
+                RPS_HIP(rps::launch_reset(d_state, ctx->compute));
+            }
             RPS_HIP(hipEventRecord(s.k_start, ctx->compute));
             RPS_HIP(var->launch_multi(M, grid, ctx->compute));
             RPS_HIP(hipEventRecord(s.k_end, ctx->compute));
-            RPS_HIP(hipMemcpyAsync(s.h_state, s.d_state, sizeof(rps::DeviceState),
+            RPS_HIP(hipMemcpyAsync(ctx->h_ctrl.ptr, ctrl, ctrl_bytes, hipMemcpyDeviceToHost,
+                                   ctx->compute));
+            RPS_HIP(hipMemcpyAsync(ctx->h_num, s.d_num, spec * sizeof(long long),
                                    hipMemcpyDeviceToHost, ctx->compute));
-            RPS_HIP(hipMemcpyAsync(ctx->h_head, ctx->m_head.ptr, sizeof(unsigned long long),
+            RPS_HIP(hipMemcpyAsync(ctx->h_den, s.d_den, spec * sizeof(int),
+                                   hipMemcpyDeviceToHost, ctx->compute));
+            RPS_HIP(hipMemcpyAsync(ctx->h_req, ctx->m_req.ptr, spec * sizeof(int),
                                    hipMemcpyDeviceToHost, ctx->compute));
             RPS_HIP(hipEventRecord(s.ready, ctx->compute));
             wait_event(s.ready, ctx->launch_timeout_ms,
                        "many-curve launch " + std::to_string(st.launches + 1) + " (tiles "
                            + std::to_string(head) + ".." + std::to_string(total_tiles)
                            + " of " + std::to_string(n) + " requests)");
-            head = *ctx->h_head;
+            if (!st.launches) {
+                guard.built = true;
+                float bms = 0.0f;
+                RPS_HIP(hipEventElapsedTime(&bms, ctx->basis_start, ctx->basis_end));
+                st.basis_ms = bms;
+            }
+            std::memcpy(&head, ctx->h_ctrl.ptr + ControlLayout::kHead, sizeof head);
             ++st.launches;
             float kms = 0.0f;
             RPS_HIP(hipEventElapsedTime(&kms, s.k_start, s.k_end));
             st.sieve_ms += kms;
-            const rps::DeviceState state = *s.h_state;
+            rps::DeviceState state;
+            std::memcpy(&state, ctx->h_ctrl.ptr + ControlLayout::kState, sizeof state);
             const unsigned long long valid = std::min(state.count, state.first_fail);
             if (valid > ctx->capacity) {
                 throw ApiError{RPS_EINTERNAL, "streaming survivor count exceeds capacity"};
             }
-            for (unsigned long long at = 0; at < valid;) {
+            keep(static_cast<size_t>(std::min<unsigned long long>(valid, spec)));
+            for (unsigned long long at = spec; at < valid;) {
                 const size_t k = static_cast<size_t>(std::min<unsigned long long>(
                     valid - at, ctx->stage_capacity));
                 RPS_HIP(hipMemcpyAsync(ctx->h_num, s.d_num + at, k * sizeof(long long),
@@ -1424,14 +1513,7 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
                 RPS_HIP(hipMemcpyAsync(ctx->h_req, ctx->m_req.ptr + at, k * sizeof(int),
                                        hipMemcpyDeviceToHost, ctx->copy));
                 RPS_HIP(hipStreamSynchronize(ctx->copy));
-                for (size_t i = 0; i < k; ++i) {
-                    const int r = ctx->h_req[i];
-                    if (r < 0 || r >= n) {
-                        throw ApiError{RPS_EINTERNAL, "survivor with an invalid request index"};
-                    }
-                    kept_num[r].push_back(ctx->h_num[i]);
-                    kept_den[r].push_back(ctx->h_den[i]);
-                }
+                keep(k);
                 at += k;
             }
             if (head > total_tiles) {
@@ -1439,8 +1521,7 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
             }
         }
         std::vector<int> failed(n);
-        RPS_HIP(hipMemcpy(failed.data(), ctx->m_failed.ptr, n * sizeof(int),
-                          hipMemcpyDeviceToHost));
+        std::memcpy(failed.data(), ctx->h_ctrl.ptr + ControlLayout::kFailed, n * sizeof(int));
         const double multi_kernel_ms = st.sieve_ms;
         const uint64_t multi_launches = st.launches;
 

@@ -858,12 +858,16 @@ struct RowTileQ {
         // Iterations whose words all lie below min(we, last word) need no
         // masking; they run the unmasked body.  The first iteration of a
         // resumed sweep and the range's tail run the masked one.
-        const uint64_t last_word = L.words - 1;
-        const uint32_t n_iter = static_cast<uint32_t>((we - ws + kStride - 1) / kStride);
+        // The bounds are wave-uniform; saying so keeps the loops scalar
+        // when the line descriptor was read from global memory (the
+        // many-curve kernel), where the compiler cannot prove it.
+        const uint64_t last_word = uniform64(L.words - 1);
+        const uint32_t n_iter =
+            uniform(static_cast<uint32_t>((we - ws + kStride - 1) / kStride));
         const uint64_t clean_end = we < last_word ? we : last_word;
         uint32_t n_clean = clean_end > ws
             ? static_cast<uint32_t>((clean_end - ws) / kStride) : 0u;
-        n_clean = n_clean < n_iter ? n_clean : n_iter;
+        n_clean = uniform(n_clean < n_iter ? n_clean : n_iter);
         const Line ln{line, static_cast<uint32_t>(slot) << kSlotShift};
         // Iteration it starts at word ws + it kStride, rel0 + it kStride from ts.
         const uint32_t rel0 = static_cast<uint32_t>(ws - ts);
@@ -1429,6 +1433,41 @@ __global__ void reset_state_kernel(DeviceState *state) {
     state->pad = 0;
 }
 
+// The state, tile counter and failure flags of a many-curve call, cleared by
+// a kernel (not a memset) so that the call's device work can be recorded in
+// a graph.
+__global__ void reset_many_kernel(DeviceState *state, unsigned long long *head, int *failed,
+                                  int n) {
+    const int stride = static_cast<int>(gridDim.x * blockDim.x);
+    for (int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) {
+        failed[i] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        state->count = 0;
+        state->first_fail = ~0ULL;
+        state->full = 0;
+        state->pad = 0;
+        *head = 0;
+    }
+}
+
+// A curve descriptor read through the constant address space (scalar loads
+// into SGPRs; the descriptors are not written while a kernel runs).
+__device__ __forceinline__ CurveDesc scalar_curve(const CurveDesc *at) {
+    static_assert(sizeof(CurveDesc) % 4 == 0, "whole dwords");
+    constexpr int kDwords = static_cast<int>(sizeof(CurveDesc) / 4);
+    const auto *src = reinterpret_cast<const __attribute__((address_space(4))) uint32_t *>(
+        reinterpret_cast<uintptr_t>(at));
+    uint32_t w[kDwords];
+#pragma unroll
+    for (int i = 0; i < kDwords; ++i) {
+        w[i] = src[i];
+    }
+    CurveDesc c;
+    __builtin_memcpy(&c, w, sizeof c);
+    return c;
+}
+
 // ---------------------------------------------------------------------------
 // Plain launches (one curve, one denominator batch)
 // ---------------------------------------------------------------------------
@@ -1647,7 +1686,11 @@ __global__ __launch_bounds__(WAVES * 64, 4) void multi_kernel(
             break;
         }
         if (!skip) {
-            const CurveDesc &C = M.curves[c];
+            // The descriptor is read through the constant address space:
+            // scalar loads into SGPRs, so every value derived from it is
+            // known to be wave-uniform and the sweep's loops stay scalar
+            // (a vector load left them as per-lane loops).
+            const CurveDesc C = scalar_curve(M.curves + c);
             const LineDesc &L = C.line;
             const unsigned long long local = tile - C.tile_start;
             bool failed = false;
@@ -1674,9 +1717,8 @@ __global__ __launch_bounds__(WAVES * 64, 4) void multi_kernel(
                     if (qs < qe
                         && Direct::sweep(pool, tk, qs, qe, L, M.out, C.request, lane) != qe) {
                         failed = true;
-                        if (lane == 0) {
-                            s_fail = 1;
-                        }
+                        s_fail = 1;  // every lane: a lane-0 branch here
+                                     // made the sweep loops per-lane
                     }
                 }
             } else {
@@ -1712,9 +1754,8 @@ __global__ __launch_bounds__(WAVES * 64, 4) void multi_kernel(
                                             static_cast<uint32_t>(r), Run::slot_of(r),
                                             unused_cursor)) {
                         failed = true;
-                        if (lane == 0) {
-                            s_fail = 1;
-                        }
+                        s_fail = 1;  // every lane: a lane-0 branch here
+                                     // made the sweep loops per-lane
                     }
                     // Queued words of the line whose ring slot line r + 2
                     // takes are drained first; s_fail, set before the
@@ -1723,9 +1764,8 @@ __global__ __launch_bounds__(WAVES * 64, 4) void multi_kernel(
                                                 static_cast<uint32_t>(r + 2), false,
                                                 unused_cursor)) {
                         failed = true;
-                        if (lane == 0) {
-                            s_fail = 1;
-                        }
+                        s_fail = 1;  // every lane: a lane-0 branch here
+                                     // made the sweep loops per-lane
                     }
                     __syncthreads();
                 }
@@ -1872,6 +1912,15 @@ void *scratch_address(int device) {
 
 hipError_t launch_reset(DeviceState *state, hipStream_t stream) {
     hipLaunchKernelGGL(reset_state_kernel, dim3(1), dim3(1), 0, stream, state);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_many(DeviceState *state, unsigned long long *head, int *failed, int n,
+                             hipStream_t stream) {
+    const unsigned blocks = static_cast<unsigned>(
+        n <= 0 ? 1 : (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024);
+    hipLaunchKernelGGL(reset_many_kernel, dim3(blocks), dim3(256), 0, stream, state, head,
+                       failed, n);
     return hipGetLastError();
 }
 

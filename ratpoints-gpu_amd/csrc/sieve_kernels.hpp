@@ -38,6 +38,9 @@ hipError_t launch_resid(const int *primes, const unsigned long long *chi,
                         int curves, const int *slot, unsigned long long *resid,
                         hipStream_t stream);
 hipError_t launch_reset(DeviceState *state, hipStream_t stream);
+// Clears a many-curve call's state, tile counter and n failure flags.
+hipError_t launch_reset_many(DeviceState *state, unsigned long long *head, int *failed, int n,
+                             hipStream_t stream);
 
 // Short-line kernel: tiles of up to kDirectMaxDenominators whole lines of at
 // most kDirectMaxWords words.

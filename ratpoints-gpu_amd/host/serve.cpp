@@ -6,15 +6,18 @@
 //     chunk of every active request's denominators in one rps_sieve_many
 //     call) and hands each request's survivors to the worker pool;
 //   * the worker pool verifies those survivors with GMP while the next
-//     round is already on the GPU, and runs the admission checks (request
-//     limits, the squarefree test, the point at infinity) of new requests,
-//     so a slow check never stalls the GPU;
+//     round is already on the GPU (a chunk of a few survivors is verified
+//     on the scheduling thread, cheaper than the handoff), and runs the
+//     admission checks (request limits, the squarefree test, the point at
+//     infinity) of new requests with large coefficients, so a slow check
+//     never stalls the GPU; small ones are checked on submit()'s thread;
 //   * the scheduling thread applies verified chunks in denominator order per
 //     request and reports finished requests, so results are exactly those of
 //     the same search alone.
 #include "serve.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -42,6 +45,9 @@ constexpr double kTableBytes = 36.0 * 512 * 9 * 8;  // residue table per curve
 constexpr double kSurvivorBytes = 16.0;              // num + den + request index
 constexpr double kSurvivorRate = 1.0 / (1ULL << 30); // generous survivors per site
 constexpr int kMaxChunksInFlight = 2;                // sieved, not yet applied
+constexpr size_t kInlineVerify = 32;                 // survivors verified on the scheduling thread
+constexpr int kSpinMicros = 200;                     // scheduling thread's poll before it sleeps
+constexpr size_t kInlineCheckChars = 4096;           // coefficient text checked on submit()'s thread
 
 double line_sites(const ServeRequest &r) {
     return 2.0 * static_cast<double>(r.options.numerator_bound) + 1.0;
@@ -190,6 +196,7 @@ struct Server::Impl {
     // Shared with submit(), the workers and the scheduling thread.
     mutable std::mutex mutex;
     std::condition_variable wake;   // the scheduling thread has work
+    std::atomic<unsigned long long> events{0};  // bumped with every wake
     std::condition_variable idle;   // every submitted request finished
     size_t outstanding = 0;
     bool stopping = false;
@@ -216,6 +223,20 @@ struct Server::Impl {
 
     double now() const {
         return std::chrono::duration<double, std::milli>(Clock::now() - start).count();
+    }
+
+    void signal() {
+        events.fetch_add(1, std::memory_order_release);
+        wake.notify_all();
+    }
+
+    // Before the scheduling thread sleeps it polls for a short while: work
+    // that arrives within kSpinMicros is taken without a thread wake-up.
+    void spin_for_work(unsigned long long seen) {
+        const auto until = Clock::now() + std::chrono::microseconds(kSpinMicros);
+        while (events.load(std::memory_order_acquire) == seen && Clock::now() < until) {
+            std::this_thread::yield();
+        }
     }
 
     void report(ServeResult &r) {
@@ -256,7 +277,7 @@ struct Server::Impl {
                 refused.push_back(std::move(failed));
             }
         }
-        wake.notify_all();
+        signal();
     }
 
     // Worker side of verification: one chunk of one request.
@@ -274,7 +295,7 @@ struct Server::Impl {
             std::lock_guard<std::mutex> lock(mutex);
             verified.push_back({std::move(a), seq, std::move(kept), sites});
         }
-        wake.notify_all();
+        signal();
     }
 
     // Moves checked requests onto the device while a slot is free and their
@@ -349,10 +370,12 @@ struct Server::Impl {
                      active.end());
     }
 
-    // One launch over a chunk of every schedulable request.  Shares are
-    // water-filled over the round's sites: requests whose remaining work is
-    // below an equal share get all of it (so short and newly arrived
-    // requests finish in the round they join), the rest split what is left.
+    // One launch over a chunk of every schedulable request.  Requests take
+    // the round's sites in order of least remaining work (shortest remaining
+    // first): short and newly arrived requests finish in the round they
+    // join, and of two long requests the one closer to its end keeps the
+    // GPU, so neither waits longer than it would behind the other one at a
+    // time.  Every scheduled request gets at least one denominator.
     void round() {
         std::vector<std::shared_ptr<Active>> take;
         for (const auto &a : active) {
@@ -377,9 +400,9 @@ struct Server::Impl {
         for (size_t i = 0; i < n; ++i) {
             Active &a = *take[i];
             const double per_line = line_sites(a.request);
-            const double share = std::max(budget, 0.0) / static_cast<double>(n - i);
             const long long left = a.last_t - a.next_t + 1;
-            const long long want = static_cast<long long>(std::floor(share / per_line));
+            const long long want = static_cast<long long>(
+                std::min(std::floor(std::max(budget, 0.0) / per_line), 9.0e18));
             const long long k = std::max(1LL, std::min(left, want));
             first[i] = a.next_t;
             last[i] = a.next_t + k - 1;
@@ -432,6 +455,12 @@ struct Server::Impl {
             a->next_t = last[i] + 1;
             const double chunk_sites = line_sites(a->request)
                 * static_cast<double>(last[i] - first[i] + 1);
+            if (found[i].numerators.size() <= kInlineVerify) {
+                // A handful of survivors: verified at once, which is quicker
+                // than handing them to a worker.
+                verify(std::move(a), seq, std::move(found[i]), chunk_sites);
+                continue;
+            }
             pool->post([this, a, seq, batch = std::move(found[i]), chunk_sites]() mutable {
                 verify(std::move(a), seq, std::move(batch), chunk_sites);
             });
@@ -450,11 +479,20 @@ struct Server::Impl {
             rps_set_option(ctx, "launch_timeout_ms", std::max(1LL, options.launch_timeout_ms));
         }
         std::deque<std::shared_ptr<Active>> incoming;
+        unsigned long long seen = ~0ULL;
         for (;;) {
             std::deque<ServeResult> failed;
             std::deque<Done> done;
+            const bool runnable =
+                (!incoming.empty() && static_cast<int>(active.size()) < options.max_active)
+                || std::any_of(active.begin(), active.end(),
+                               [](const std::shared_ptr<Active> &a) { return a->schedulable(); });
+            if (!runnable) {
+                spin_for_work(seen);
+            }
             {
                 std::unique_lock<std::mutex> lock(mutex);
+                seen = events.load(std::memory_order_acquire);
                 auto busy = [&] {
                     for (const auto &a : active) {
                         if (a->schedulable()) {
@@ -513,7 +551,7 @@ Server::~Server() {
         std::lock_guard<std::mutex> lock(impl_->mutex);
         impl_->stopping = true;
     }
-    impl_->wake.notify_all();
+    impl_->signal();
     impl_->scheduler.join();
     impl_->pool.reset();
 }
@@ -524,6 +562,17 @@ void Server::submit(ServeRequest request) {
         std::lock_guard<std::mutex> lock(impl_->mutex);
         ++impl_->outstanding;
         ++impl_->checking;
+    }
+    // Small polynomials are checked on the caller's thread (microseconds,
+    // and no handoff); large ones, whose squarefree test can take longer, on
+    // the worker pool.  Neither runs on the scheduling thread.
+    size_t chars = 0;
+    for (const std::string &c : request.options.coefficients) {
+        chars += c.size();
+    }
+    if (chars <= kInlineCheckChars) {
+        impl_->check(std::move(request), at);
+        return;
     }
     Impl *impl = impl_.get();
     impl_->pool->post([impl, request = std::move(request), at]() mutable {

@@ -73,7 +73,9 @@ public:
     Server(const Server &) = delete;
     Server &operator=(const Server &) = delete;
 
-    void submit(ServeRequest request);  // thread-safe
+    // Thread-safe.  The admission checks of a request with short
+    // coefficients run before submit returns, the others on a worker.
+    void submit(ServeRequest request);
     void drain();                       // until every submitted request finished
     double now_ms() const;
     ServeStats stats() const;

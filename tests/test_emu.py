@@ -199,3 +199,39 @@ def test_mode_coverage_trace_through_serving_loop(workloads, tmp_path):
         assert proc.returncode == 0, proc.stderr
         results[mode] = test_serve.parse_results(out)
     test_serve.check(reqs, results, oracle_limit=10 ** 9)
+
+
+def test_mode_coverage_trace_at_full_line_shapes(workloads, tmp_path):
+    # The mode-coverage serving trace with its full line shapes: square-
+    # denominator cubics and quartics at A = 1e5 (lines of 3,126 words, row
+    # runs) and degree 50-100 curves at height 3,000 (lines of 94 words,
+    # short-line tiles) share every many-curve launch, with the tiny curves
+    # besides; fewer denominators than the full trace (k <= 200, b <= 300)
+    # and several rounds.  Every launch must finish, with the same points in
+    # both modes and the oracle's points for the tiny requests.
+    if not os.path.exists(EMU_SERVE):
+        pytest.skip("ratpoints_serve_emu not built")
+    import test_serve
+    reqs = test_serve.mode_trace(workloads, 24, square_k=200, square_bound=100_000,
+                                 high_height=3000)
+    for r in reqs:
+        if not r.get("square"):
+            r["last"] = 300
+    small = test_serve.mode_trace(workloads, 8, square_k=40, square_bound=60, high_height=12)
+    for k, r in enumerate(small):
+        r["id"] = 100 + k
+    trace = tmp_path / "trace.txt"
+    workloads.write_trace(str(trace), reqs + small)
+    env = dict(os.environ, **SCHEDULE)
+    results = {}
+    for mode in ("serve", "alone"):
+        out = tmp_path / f"{mode}.txt"
+        proc = subprocess.run([EMU_SERVE, "--trace", str(trace), "--mode", mode, "--out",
+                               str(out), "--round-sites", "1e8", "--max-active", "32",
+                               "--progress"],
+                              capture_output=True, text=True, timeout=900, env=env)
+        assert proc.returncode == 0, proc.stderr
+        if mode == "serve":
+            assert proc.stderr.count("serve: round") >= 3, proc.stderr
+        results[mode] = test_serve.parse_results(out)
+    test_serve.check(reqs + small, results, oracle_limit=10_000)
