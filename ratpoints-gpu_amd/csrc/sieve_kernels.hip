@@ -74,6 +74,25 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
     return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
+// The wave's issue priority (0..3; the SIMD arbiter prefers higher
+// priority, then older waves).
+__device__ __forceinline__ void set_priority(int level) {
+    switch (level) {
+    case 3:
+        __builtin_amdgcn_s_setprio(3);
+        break;
+    case 2:
+        __builtin_amdgcn_s_setprio(2);
+        break;
+    case 1:
+        __builtin_amdgcn_s_setprio(1);
+        break;
+    default:
+        __builtin_amdgcn_s_setprio(0);
+        break;
+    }
+}
+
 // Orders one wave's LDS accesses before and after this point (lanes read
 // what other lanes of the wave wrote).
 __device__ __forceinline__ void wave_sync() {
@@ -880,13 +899,37 @@ struct RowTileQ {
             }
             it = 1;
         }
+        // Wave priority falls with the wave's progress through its range:
+        // 3 in its first quarter down to 0 in its last.  The waves of a
+        // workgroup sweep equal ranges, but the SIMDs issue the oldest waves
+        // first, so without this the youngest wave on each SIMD finished
+        // its range last (on 156,251-word lines the waves took 61k to 99k
+        // clocks for equal work, and the line ended with the slowest); a
+        // wave behind the others now takes the issue slots first.
+        set_priority(3);
+        const uint32_t quarter = (n_clean - it + 3) / 4;
+        uint32_t mark = it + quarter;
+        uint32_t stage = 0;
         for (; it < n_clean; ++it) {
+            if (it == mark) {
+                ++stage;
+                mark += quarter;
+                if (stage == 1) {
+                    set_priority(2);
+                } else if (stage == 2) {
+                    set_priority(1);
+                } else {
+                    set_priority(0);
+                }
+            }
             if (!step<false>(pool, 0, rel0 + it * kStride, we, last_word, 0u, o, q, ln, wave,
                              ts, L, O, request, lane)) {
+                set_priority(0);
                 fail = oldest(pool, wave, q, ts);
                 return false;
             }
         }
+        set_priority(0);
         for (; it < n_iter; ++it) {
             if (!step<true>(pool, ws + static_cast<uint64_t>(it) * kStride, rel0 + it * kStride,
                             we, last_word, 0u, o, q, ln, wave, ts, L, O, request, lane)) {

@@ -137,6 +137,7 @@ inline int __ffsll(long long x) { return __builtin_ffsll(x); }
 #define __builtin_amdgcn_wave_barrier() ((void)::emu::wave_op(::emu::kWaveSync, 0, 0, __LINE__))
 #define __builtin_amdgcn_fence(order, scope) ((void)0)
 #define __builtin_amdgcn_s_sleep(n) ((void)0)
+#define __builtin_amdgcn_s_setprio(n) ((void)0)
 
 inline uint32_t __builtin_amdgcn_mbcnt_lo(uint32_t mask, uint32_t acc) {
     const int l = ::emu::lane_id();
