@@ -182,9 +182,13 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests,
  * out of range.  No counterpart in the reference, whose knobs were
  * compile-time macros (src/sieve_plan.hpp:10-22, src/sieve.cu:15-17):
  *   "survivor_capacity"  survivors per flush buffer, >= 16384 (default 2^24)
- *   "batch_size"         denominators per launch, 1..1048576 (default 65536)
+ *   "batch_size"         denominators per launch, 1..1048576 (default 65536;
+ *                        until set, rps_tuned's choice where it has one)
  *   "variant"            kernel variant index, 0..rps_variant_count()-1, or
- *                        -1 (default): chosen per request by rps_tuned_variant
+ *                        -1 (default): chosen per request by rps_tuned
+ *   "run_log2"           row tiles take whole lines up to 2^run_log2 words,
+ *                        10..24, or -1 (default): rps_tuned's, or a forced
+ *                        variant's own
  *   "min_tiles"          split denominators into word chunks until at least
  *                        this many tiles per launch exist (default 2048)
  *   "direct_words"       lines of at most this many 64-bit words use the
@@ -215,6 +219,24 @@ const char *rps_variant_name(int variant);
  * (used when the "variant" option is -1, the default).  No counterpart in the
  * reference, whose kernel has one configuration. */
 int rps_tuned_variant(uint64_t words);
+
+/* The autotuner's full choice for lines of `words` words on the plain path
+ * (many == 0: rps_sieve and its streaming forms) or for many-curve launches
+ * (many != 0): variant, run length (row tiles take whole lines up to
+ * 2^run_log2 words) and denominators per launch (0: the "batch_size"
+ * option's; many-curve launches have none).  Used when the "variant",
+ * "run_log2" and "batch_size" options are left at their defaults.  The
+ * reference's counterpart is its tuning script's BLOCK / NUM_PRIMES /
+ * INITIAL_PRIMES sweep (tests/tune_cuda.py:90-158), whose result is compiled
+ * in by hand. */
+int rps_tuned(uint64_t words, int many, int *variant, int *run_log2, int *batch);
+
+/* Shape of a compiled variant: 64-bit words per lane per iteration, dense
+ * (row-table) primes, waves per workgroup, default run length, and whether
+ * its rows hold 16-byte elements (one workgroup per CU).  RPS_EINVAL for an
+ * index out of range.  No counterpart in the reference. */
+int rps_variant_info(int variant, int *words_per_lane, int *dense_primes, int *waves,
+                     int *run_log2, int *wide);
 
 /* Device enumeration for --list-devices (include/gpu_devices.hpp:20-21,
  * src/gpu_devices.cpp:55-97).  rps_device_info writes a NUL-terminated name

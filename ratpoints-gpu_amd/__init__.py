@@ -119,6 +119,11 @@ def bind_library(path):
     lib.rps_variant_name.argtypes = [ctypes.c_int]
     lib.rps_tuned_variant.restype = ctypes.c_int
     lib.rps_tuned_variant.argtypes = [ctypes.c_uint64]
+    lib.rps_tuned.restype = ctypes.c_int
+    lib.rps_tuned.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                              ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    lib.rps_variant_info.restype = ctypes.c_int
+    lib.rps_variant_info.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 5
     lib.rps_device_count.restype = ctypes.c_int
     lib.rps_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
     lib.rps_check_request.restype = ctypes.c_int
@@ -150,6 +155,26 @@ def tuned_variant(words):
     """Index of the variant the tuning table picks for lines of `words`
     64-bit words."""
     return load_library().rps_tuned_variant(int(words))
+
+
+def tuned(words, many=False):
+    """The tuning table's (variant index, run_log2, batch) for lines of
+    `words` words on the plain path or in many-curve launches."""
+    v, r, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    load_library().rps_tuned(int(words), 1 if many else 0, ctypes.byref(v), ctypes.byref(r),
+                             ctypes.byref(b))
+    return v.value, r.value, b.value
+
+
+def variant_info(index, lib=None):
+    """Shape of a compiled variant: dict of words_per_lane, dense_primes,
+    waves, run_log2, wide."""
+    lib = lib or load_library()
+    vals = [ctypes.c_int() for _ in range(5)]
+    if lib.rps_variant_info(int(index), *[ctypes.byref(v) for v in vals]) != 0:
+        raise ValueError("no variant %d" % index)
+    return dict(zip(("words_per_lane", "dense_primes", "waves", "run_log2", "wide"),
+                    (v.value for v in vals)))
 
 
 class SieveResult:

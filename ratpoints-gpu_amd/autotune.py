@@ -1,11 +1,23 @@
-"""Autotuner: picks the row-tile kernel variant per line-length bucket.
+"""Autotuner: picks the launch shape per line-length bucket.
 
 Runs on a GPU.  For each bucket of line lengths (64-bit words per
-denominator line) it times every compiled variant on the plain path over
-one representative line length (a planted sextic, about 3e11 sites, runs of
-lines on), and writes ``csrc/tuning_table.inc`` -- the table the library
-uses to choose a variant for a request when the caller has not forced one
-with ``rps_set_option("variant")``.  Rebuild the library afterwards.
+denominator line) it times, over one representative line length of a
+planted sextic (about 3e11 sites per point):
+
+  * on the plain path (rps_sieve): every compiled variant -- dense-prime
+    count, words per lane, waves per workgroup, row width -- at three run
+    lengths each (whole lines per row tile up to 2^run_log2 words: the
+    variant's own and one step either side), by kernel time; then, for the
+    fastest, three batch sizes (denominators per launch) by wall time;
+  * on the many-curve path (rps_sieve_many, the search split into 64
+    requests of equal shape in one call): every variant at the same run
+    lengths, by kernel time;
+
+and writes ``csrc/tuning_table.inc``, the table the library uses when the
+caller leaves the "variant", "run_log2" and "batch_size" options at their
+defaults (``rps_tuned``).  Rebuild the library afterwards.  The reference's
+counterpart is tests/tune_cuda.py:90-158 (a BLOCK / NUM_PRIMES /
+INITIAL_PRIMES sweep whose result is compiled in by hand).
 
     python ratpoints-gpu_amd/autotune.py            # measure and write
     python ratpoints-gpu_amd/autotune.py --dry-run  # measure and print
@@ -26,6 +38,8 @@ TABLE = os.path.join(HERE, "csrc", "tuning_table.inc")
 # Upper ends of the buckets (words per line) and the length timed for each.
 BUCKETS = [(1536, 1024), (3072, 2048), (6144, 4096), (12288, 8192),
            (24576, 16384), (49152, 32768), (98304, 65536), (1 << 62, 156251)]
+BATCHES = (32768, 65536, 131072)
+MANY_REQUESTS = 64
 
 
 def load(name, path):
@@ -36,44 +50,88 @@ def load(name, path):
     return module
 
 
+def run_lengths(pkg, v):
+    own = pkg.variant_info(v)["run_log2"]
+    return [r for r in (own - 1, own, own + 1) if 10 <= r <= 24]
+
+
+def many_requests(coeffs, bound, lines):
+    per = max(1, lines // MANY_REQUESTS)
+    return [{"coefficients": coeffs, "bound": bound, "first": 1 + k * per,
+             "last": (k + 1) * per} for k in range(MANY_REQUESTS)]
+
+
 def measure(sites_per_point=3e11):
     pkg = load("ratpoints_gpu_amd", os.path.join(HERE, "__init__.py"))
     wl = load("ratpoints_gpu_amd_workloads", os.path.join(HERE, "workloads.py"))
     coeffs = [str(c) for c in wl.sextic_a5e6_curve()[0]]
     names = pkg.variants()
     rows = []
+    no_sink = lambda n, d: False  # noqa: E731
     with pkg.SieveContext(0) as ctx:
         for upper, words in BUCKETS:
             bound = 32 * words - 32
             lines = max(65536, int(sites_per_point / (2 * bound + 1)) // 65536 * 65536)
-            rates = {}
+            row = {"max_words": upper, "words": words, "plain": {}, "many": {}, "batch": {}}
+            ctx.set_option("batch_size", 65536)
             for v, name in enumerate(names):
                 ctx.set_option("variant", v)
-                ctx.run(coeffs, bound, 1, 65536, False, lambda n, d: False)
-                ctx.run(coeffs, bound, 1, lines, False, lambda n, d: False)
-                ms, _, sites = ctx.kernel_timing()
-                rates[name] = sites / ms / 1e6
-            best = max(rates, key=rates.get)
-            row = {"max_words": upper, "words": words, "best": best,
-                   "gsites": {k: round(r, 1) for k, r in rates.items()}}
+                for r in run_lengths(pkg, v):
+                    ctx.set_option("run_log2", r)
+                    ctx.run(coeffs, bound, 1, 65536, False, no_sink)
+                    ctx.run(coeffs, bound, 1, lines, False, no_sink)
+                    ms, _, sites = ctx.kernel_timing()
+                    row["plain"]["%s/%d" % (name, r)] = round(sites / ms / 1e6, 1)
+                    reqs = many_requests(coeffs, bound, lines)
+                    ctx.run_many(reqs[:2], lambda q, n, d: False)
+                    ctx.run_many(reqs, lambda q, n, d: False)
+                    ms, _, sites = ctx.kernel_timing()
+                    row["many"]["%s/%d" % (name, r)] = round(sites / ms / 1e6, 1)
+            best = max(row["plain"], key=row["plain"].get)
+            name, r = best.split("/")
+            ctx.set_option("variant", names.index(name))
+            ctx.set_option("run_log2", int(r))
+            for batch in BATCHES:
+                ctx.set_option("batch_size", batch)
+                ctx.run(coeffs, bound, 1, lines, False, no_sink)
+                t0 = time.perf_counter()
+                ctx.run(coeffs, bound, 1, lines, False, no_sink)
+                wall = time.perf_counter() - t0
+                _, _, sites = ctx.kernel_timing()
+                row["batch"][str(batch)] = round(sites / wall / 1e9, 1)
+            row["best"] = best
+            row["best_batch"] = int(max(row["batch"], key=row["batch"].get))
+            row["best_many"] = max(row["many"], key=row["many"].get)
             print(json.dumps(row), flush=True)
             rows.append(row)
+            ctx.set_option("variant", -1)
+            ctx.set_option("run_log2", -1)
     return rows
 
 
 def write_table(rows, path=TABLE):
     lines = [
-        "// Generated by ratpoints-gpu_amd/autotune.py on one MI355X (%s);" % time.strftime("%Y-%m-%d"),
-        "// kernel Gsite/s of every variant per line-length bucket, the fastest chosen.",
-        "// {largest words per line of the bucket, variant name}",
+        "// Generated by ratpoints-gpu_amd/autotune.py on one MI355X (%s): kernel" % time.strftime("%Y-%m-%d"),
+        "// Gsite/s of every variant/run_log2 per line-length bucket on the plain",
+        "// path and in many-curve launches, and end-to-end Gsite/s of the plain",
+        "// path's fastest shape per batch size; the fastest of each chosen.",
+        "// {largest words per line of the bucket, plain variant, run_log2, batch,",
+        "//  many-curve variant, run_log2}",
     ]
     for row in rows:
-        rates = ", ".join("%s %.0f" % (k, v) for k, v in sorted(row["gsites"].items()))
-        lines.append("// <= %d words (timed at %d): %s" % (min(row["max_words"], 10 ** 12),
-                                                           row["words"], rates))
+        upper = min(row["max_words"], 10 ** 12)
+        for kind in ("plain", "many"):
+            rates = ", ".join("%s %.0f" % (k, v) for k, v in sorted(row[kind].items()))
+            lines.append("// <= %d words (timed at %d), %s: %s" % (upper, row["words"], kind,
+                                                                  rates))
+        rates = ", ".join("%s %.0f" % (k, v) for k, v in row["batch"].items())
+        lines.append("// <= %d words, plain %s by batch: %s" % (upper, row["best"], rates))
     for row in rows:
         upper = "~0ULL" if row["max_words"] >= 1 << 62 else "%dULL" % row["max_words"]
-        lines.append('{%s, "%s"},' % (upper, row["best"]))
+        name, r = row["best"].split("/")
+        many, mr = row["best_many"].split("/")
+        lines.append('{%s, "%s", %s, %d, "%s", %s},' % (upper, name, r, row["best_batch"],
+                                                        many, mr))
     with open(path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
 

@@ -184,6 +184,15 @@ struct Grow {
         size = n;
     }
 
+    // As ensure, but at least doubling and at least `floor` elements, for
+    // per-call buffers whose size varies from call to call (each
+    // reallocation costs a device synchronisation).
+    void ensure_grow(size_t n, size_t floor) {
+        if (n > size) {
+            ensure(std::max({n, 2 * size, floor}));
+        }
+    }
+
     void release() {
         if (ptr) {
             (void)hipFree(ptr);
@@ -210,6 +219,14 @@ struct PinnedGrow {
         }
         RPS_HIP(hipHostMalloc(reinterpret_cast<void **>(&ptr), n * sizeof(T), 0));
         size = n;
+    }
+
+    // At least doubling and at least `floor` elements (pinned allocations
+    // take milliseconds).
+    void ensure_grow(size_t n, size_t floor) {
+        if (n > size) {
+            ensure(std::max({n, 2 * size, floor}));
+        }
     }
 
     void release() {
@@ -284,7 +301,9 @@ struct rps_ctx {
     int *h_req = nullptr;
     size_t stage_capacity = size_t(1) << 20;
     int batch_size = 1 << 16;
+    bool batch_forced = false;  // batch_size set by the caller: the table's is not used
     int variant = -1;  // -1: chosen per request by the tuning table
+    int run_log2 = -1;  // -1: the tuning table's (or a forced variant's own) run length
     long long min_tiles = 2048;
     unsigned long long direct_words = rps::kDirectMaxWords;
     int primitive = 0;  // emit only survivors with gcd(a, b) = 1
@@ -394,30 +413,77 @@ void validate(const rps_request *req) {
     }
 }
 
-// Variant per line length from the autotuner's table (autotune.py writes
-// tuning_table.inc); the first entry whose bucket holds `words`, or variant 0
-// when a name in the table is not compiled in.
+// Launch shape per line length from the autotuner's table (autotune.py
+// writes tuning_table.inc), for the plain path (rps_sieve and its streaming
+// forms) and for many-curve launches (rps_sieve_many): the tile variant
+// (dense primes, words per lane, waves per workgroup, row width), the run
+// length (whole lines per row tile up to 2^run_log2 words) and, for the
+// plain path, the denominators per launch.  The first entry whose bucket
+// holds `words` applies; a name that is not compiled in falls back to
+// variant 0 with its own run length.
 struct TuneEntry {
     unsigned long long max_words;
     const char *variant;
+    int run_log2;
+    int batch;
+    const char *many_variant;
+    int many_run_log2;
 };
 
 const TuneEntry kTuning[] = {
 #include "tuning_table.inc"
 };
 
-int tuned_variant(unsigned long long words) {
-    for (const TuneEntry &t : kTuning) {
-        if (words <= t.max_words) {
-            for (int v = 0; v < rps::variant_count(); ++v) {
-                if (std::strcmp(rps::variant(v)->name, t.variant) == 0) {
-                    return v;
-                }
-            }
-            return 0;
+struct Tuned {
+    int variant = 0;
+    int run_log2 = 0;
+    int batch = 0;  // 0: the context's batch_size
+};
+
+int variant_index(const char *name) {
+    for (int v = 0; v < rps::variant_count(); ++v) {
+        if (std::strcmp(rps::variant(v)->name, name) == 0) {
+            return v;
         }
     }
-    return 0;
+    return -1;
+}
+
+Tuned tuned(unsigned long long words, bool many) {
+    Tuned t;
+    t.run_log2 = rps::variant(0)->run_log2;
+    for (const TuneEntry &e : kTuning) {
+        if (words <= e.max_words) {
+            const int v = variant_index(many ? e.many_variant : e.variant);
+            if (v >= 0) {
+                t.variant = v;
+                t.run_log2 = many ? e.many_run_log2 : e.run_log2;
+                t.batch = many ? 0 : e.batch;
+            }
+            break;
+        }
+    }
+    return t;
+}
+
+int tuned_variant(unsigned long long words) { return tuned(words, false).variant; }
+
+// What a request runs with: the caller's options where set, else the tuning
+// table's choice (a forced variant keeps its own run length).
+struct Shape {
+    const rps::VariantInfo *var;
+    int run_log2;
+    int batch;
+};
+
+Shape launch_shape(const rps_ctx *ctx, unsigned long long words, bool many, int forced = -1) {
+    const Tuned t = tuned(words, many);
+    const int chosen = forced >= 0 ? forced : ctx->variant >= 0 ? ctx->variant : t.variant;
+    Shape sh{rps::variant(chosen), 0, 0};
+    sh.run_log2 = ctx->run_log2 >= 0 ? ctx->run_log2
+                  : chosen == t.variant ? t.run_log2 : sh.var->run_log2;
+    sh.batch = ctx->batch_forced || t.batch <= 0 ? ctx->batch_size : t.batch;
+    return sh;
 }
 
 // Line geometry of a request: words per line, last-word mask, and the
@@ -695,6 +761,13 @@ int rps_set_option(rps_ctx *ctx, const char *name, int64_t value) {
             return RPS_EINVAL;
         }
         ctx->batch_size = static_cast<int>(value);
+        ctx->batch_forced = true;
+    } else if (key == "run_log2") {
+        if (value != -1 && (value < 10 || value > 24)) {
+            ctx->error = "run_log2 must be -1 or in 10..24";
+            return RPS_EINVAL;
+        }
+        ctx->run_log2 = static_cast<int>(value);
     } else if (key == "variant") {
         if (value < -1 || value >= rps::variant_count()) {
             ctx->error = "variant out of range";
@@ -741,6 +814,44 @@ int rps_set_option(rps_ctx *ctx, const char *name, int64_t value) {
 int rps_variant_count(void) { return rps::variant_count(); }
 
 int rps_tuned_variant(uint64_t words) { return tuned_variant(words); }
+
+int rps_tuned(uint64_t words, int many, int *variant, int *run_log2, int *batch) {
+    const Tuned t = tuned(words, many != 0);
+    if (variant) {
+        *variant = t.variant;
+    }
+    if (run_log2) {
+        *run_log2 = t.run_log2;
+    }
+    if (batch) {
+        *batch = t.batch;
+    }
+    return RPS_OK;
+}
+
+int rps_variant_info(int index, int *words_per_lane, int *dense_primes, int *waves,
+                     int *run_log2, int *wide) {
+    if (index < 0 || index >= rps::variant_count()) {
+        return RPS_EINVAL;
+    }
+    const rps::VariantInfo *v = rps::variant(index);
+    if (words_per_lane) {
+        *words_per_lane = v->words_per_lane;
+    }
+    if (dense_primes) {
+        *dense_primes = v->fast_primes;
+    }
+    if (waves) {
+        *waves = v->waves;
+    }
+    if (run_log2) {
+        *run_log2 = v->run_log2;
+    }
+    if (wide) {
+        *wide = v->wide;
+    }
+    return RPS_OK;
+}
 
 const char *rps_variant_name(int index) {
     const rps::VariantInfo *v = rps::variant(index);
@@ -824,8 +935,8 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
         validate(req);
         RPS_HIP(hipSetDevice(ctx->device));
         const rps::LineDesc geometry = line_geometry(req);
-        const rps::VariantInfo *var = rps::variant(
-            ctx->variant >= 0 ? ctx->variant : tuned_variant(geometry.words));
+        const Shape shape = launch_shape(ctx, geometry.words, false);
+        const rps::VariantInfo *var = shape.var;
         const int stride = 64 * var->words_per_lane;
         const unsigned long long words = geometry.words;
         const long long A = req->numerator_bound;
@@ -891,7 +1002,7 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
             if (next_t > last_t) {
                 return false;
             }
-            long long n = std::min<long long>(ctx->batch_size, last_t - next_t + 1);
+            long long n = std::min<long long>(shape.batch, last_t - next_t + 1);
             bt.line = base.line;
             if (words <= ctx->direct_words) {
                 direct_tiling(bt.line, std::max(1LL, n / std::max(1LL, ctx->min_tiles)));
@@ -905,7 +1016,7 @@ int rps_sieve_batched(rps_ctx *ctx, const rps_request *req, rps_sink sink,
                 chunks = line_chunks(ctx->min_tiles, n, words, stride, var->waves);
                 row_tiling(bt.line, chunks, stride);
                 row_runs(bt.line, std::max(1LL, n / std::max(1LL, ctx->min_tiles)),
-                         var->run_log2);
+                         shape.run_log2);
             }
             bt.line.first_t = static_cast<int>(next_t);
             bt.line.n_t = static_cast<int>(n);
@@ -1049,8 +1160,8 @@ int rps_sieve_into(rps_ctx *ctx, const rps_request *req, int64_t *numerators,
         auto *cursors = reinterpret_cast<unsigned long long *>(scratch + S::kCursors);
 
         const rps::LineDesc geometry = line_geometry(req);
-        const rps::VariantInfo *var = rps::variant(
-            ctx->variant >= 0 ? ctx->variant : tuned_variant(geometry.words));
+        const Shape shape = launch_shape(ctx, geometry.words, false);
+        const rps::VariantInfo *var = shape.var;
         const int stride = 64 * var->words_per_lane;
         const int waves = std::max(var->waves, rps::kDirectWaves);
         if (static_cast<unsigned long long>(waves) > S::kMaxWaves) {
@@ -1101,7 +1212,7 @@ int rps_sieve_into(rps_ctx *ctx, const rps_request *req, int64_t *numerators,
         RPS_HIP(rps::launch_reset(state, ctx->compute));
         const double line_sites = 2.0 * static_cast<double>(A) + 1.0;
         for (long long next_t = req->den_first; next_t <= req->den_last;) {
-            long long n = std::min<long long>(ctx->batch_size, req->den_last - next_t + 1);
+            long long n = std::min<long long>(shape.batch, req->den_last - next_t + 1);
             n = std::min<long long>(n, static_cast<long long>(S::kMaxTiles));
             rps::SieveParams prm = base;
             if (words <= ctx->direct_words) {
@@ -1113,7 +1224,7 @@ int rps_sieve_into(rps_ctx *ctx, const rps_request *req, int64_t *numerators,
                 chunks = line_chunks(ctx->min_tiles, n, words, stride, var->waves);
                 row_tiling(prm.line, chunks, stride);
                 row_runs(prm.line, std::max(1LL, n / std::max(1LL, ctx->min_tiles)),
-                         var->run_log2);
+                         shape.run_log2);
             }
             prm.line.first_t = static_cast<int>(next_t);
             prm.line.n_t = static_cast<int>(n);
@@ -1175,10 +1286,36 @@ int rps_check_request(const rps_request *request, char *message, size_t message_
 
 namespace {
 
+// Survivors of the plain path handed to a many-curve sink as request 0.
+struct SingleSink {
+    rps_multi_sink sink;
+    void *user;
+    std::vector<int32_t> zeros;
+
+    static int call(void *u, const int64_t *num, const int32_t *den, size_t count) {
+        SingleSink &s = *static_cast<SingleSink *>(u);
+        if (s.zeros.size() < count) {
+            s.zeros.assign(count, 0);
+        }
+        return s.sink(s.user, s.zeros.data(), num, den, count);
+    }
+};
+
 // One persistent launch over all the requests, with `forced` (>= 0) or the
-// tuning table's variant for the request with the most sites.
+// tuning table's variant for the request with the most sites.  A single
+// request takes the plain path instead: the same survivors, from a grid
+// and tiles sized for that one curve.
 int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
                       rps_multi_sink sink, void *user, rps_stats *stats, int forced) {
+    if (n_requests == 1 && requests && sink && forced < 0) {
+        SingleSink single{sink, user, {}};
+        const int rc = rps_sieve_batched(ctx, requests, SingleSink::call, nullptr, &single,
+                                         stats);
+        if (stats) {
+            stats->batches = 1;
+        }
+        return rc;
+    }
     rps_stats local{};
     rps_stats &st = stats ? *stats : local;
     st = rps_stats{};
@@ -1205,24 +1342,23 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
             n_coef = std::max(n_coef, requests[r].n_coefficients);
         }
         RPS_HIP(hipSetDevice(ctx->device));
-        // One variant per launch: the forced one, or the tuning table's
-        // choice for the request with the most sites.
-        int chosen = forced >= 0 ? forced : ctx->variant;
-        if (chosen < 0) {
-            int largest = 0;
-            double most = -1.0;
-            for (int r = 0; r < n; ++r) {
-                const rps::LineDesc g = line_geometry(&requests[r]);
-                const double work = (2.0 * static_cast<double>(g.bound) + 1.0)
-                    * (static_cast<double>(requests[r].den_last) - requests[r].den_first + 1.0);
-                if (work > most) {
-                    most = work;
-                    largest = r;
-                }
+        // One variant and run length per launch: the options' or the forced
+        // variant, else the tuning table's many-curve choice for the request
+        // with the most sites.
+        int largest = 0;
+        double most = -1.0;
+        for (int r = 0; r < n; ++r) {
+            const rps::LineDesc g = line_geometry(&requests[r]);
+            const double work = (2.0 * static_cast<double>(g.bound) + 1.0)
+                * (static_cast<double>(requests[r].den_last) - requests[r].den_first + 1.0);
+            if (work > most) {
+                most = work;
+                largest = r;
             }
-            chosen = tuned_variant(line_geometry(&requests[largest]).words);
         }
-        const rps::VariantInfo *var = rps::variant(chosen);
+        const Shape shape = launch_shape(ctx, line_geometry(&requests[largest]).words, true,
+                                         forced);
+        const rps::VariantInfo *var = shape.var;
         const int stride = 64 * var->words_per_lane;
 
         // Geometry and tiling per curve.  Long lines are cut into chunks of
@@ -1246,7 +1382,7 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
                     throw ApiError{RPS_EINVAL, "maximum height exceeds GPU limit"};
                 }
                 row_tiling(L, chunks, stride);
-                row_runs(L, L.n_t, var->run_log2);
+                row_runs(L, L.n_t, shape.run_log2);
             }
             lines[r] = L;
             const double line_sites = 2.0 * static_cast<double>(L.bound) + 1.0;
@@ -1333,7 +1469,8 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
             }
         } guard{ctx};
 
-        ctx->m_chi.ensure(std::max<size_t>(1, static_cast<size_t>(n_build) * rps::kCandidates * rps::kChiWords));
+        ctx->m_chi.ensure_grow(static_cast<size_t>(n_build) * rps::kCandidates * rps::kChiWords,
+                               size_t(64) * rps::kCandidates * rps::kChiWords);
         ctx->m_req.ensure(ctx->capacity);
         ensure_capacity(ctx);
         std::vector<rps::CurveDesc> curves(n);
@@ -1363,10 +1500,10 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
         const size_t coef_ints = static_cast<size_t>(n_build) * rps::kCandidates * n_coef;
         const PlanLayout plan(coef_ints, static_cast<size_t>(n_build), static_cast<size_t>(n),
                               n_buckets);
-        ctx->m_plan.ensure(plan.bytes);
-        ctx->h_plan.ensure(plan.bytes);
-        ctx->m_ctrl.ensure(ControlLayout::bytes(static_cast<size_t>(n)));
-        ctx->h_ctrl.ensure(ControlLayout::bytes(static_cast<size_t>(n)));
+        ctx->m_plan.ensure_grow(plan.bytes, size_t(1) << 20);
+        ctx->h_plan.ensure_grow(plan.bytes, size_t(1) << 20);
+        ctx->m_ctrl.ensure_grow(ControlLayout::bytes(static_cast<size_t>(n)), size_t(1) << 16);
+        ctx->h_ctrl.ensure_grow(ControlLayout::bytes(static_cast<size_t>(n)), size_t(1) << 16);
         unsigned char *hp = ctx->h_plan.ptr;
         unsigned char *dp = ctx->m_plan.ptr;
         int *h_coef = reinterpret_cast<int *>(hp + plan.coef);
@@ -1448,8 +1585,11 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
         // One workgroup per resident slot: the variant's occupancy per CU.
         int per_cu = 0;
         RPS_HIP(var->multi_blocks_per_cu(&per_cu));
-        const unsigned grid = static_cast<unsigned>(ctx->cu_count)
-            * static_cast<unsigned>(std::max(1, per_cu));
+        // No more workgroups than tiles: a small call would otherwise start
+        // every resident slot only to find the tiles handed out.
+        const unsigned grid = static_cast<unsigned>(std::min<unsigned long long>(
+            static_cast<unsigned long long>(ctx->cu_count) * std::max(1, per_cu),
+            std::max(1ULL, total_tiles)));
         const size_t spec = std::min({kSpeculativeSurvivors, ctx->stage_capacity, ctx->capacity});
         const size_t ctrl_bytes = ControlLayout::bytes(static_cast<size_t>(n));
         std::vector<std::vector<long long>> kept_num(n);
@@ -1663,8 +1803,8 @@ int sieve_many_split(rps_ctx *ctx, const rps_request *requests, size_t n_request
             validate(&requests[r]);
             int f = 0;
             if (ctx->variant < 0) {
-                const int v = tuned_variant(line_geometry(&requests[r]).words);
-                f = rps::variant(v)->wide ? 1 : 0;
+                const Tuned t = tuned(line_geometry(&requests[r]).words, true);
+                f = rps::variant(t.variant)->wide ? 1 : 0;
             }
             family[f].push_back(static_cast<int32_t>(r));
         }

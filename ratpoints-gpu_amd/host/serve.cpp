@@ -46,7 +46,7 @@ constexpr double kSurvivorBytes = 16.0;              // num + den + request inde
 constexpr double kSurvivorRate = 1.0 / (1ULL << 30); // generous survivors per site
 constexpr int kMaxChunksInFlight = 2;                // sieved, not yet applied
 constexpr size_t kInlineVerify = 32;                 // survivors verified on the scheduling thread
-constexpr int kSpinMicros = 200;                     // scheduling thread's poll before it sleeps
+constexpr int kSpinMicros = 2000000;                 // scheduling thread's poll before it sleeps
 constexpr size_t kInlineCheckChars = 4096;           // coefficient text checked on submit()'s thread
 
 double line_sites(const ServeRequest &r) {
@@ -214,6 +214,7 @@ struct Server::Impl {
 
     // Scheduling thread only.
     rps_ctx *ctx = nullptr;
+    double last_round_end = 0.0;
     std::string open_error;
     std::vector<std::shared_ptr<Active>> active;
     double used = 0.0;
@@ -230,8 +231,10 @@ struct Server::Impl {
         wake.notify_all();
     }
 
-    // Before the scheduling thread sleeps it polls for a short while: work
-    // that arrives within kSpinMicros is taken without a thread wake-up.
+    // Before the scheduling thread sleeps it polls (yielding the core) for
+    // up to kSpinMicros: under a steady stream of requests it never sleeps,
+    // so a request that arrives at an idle GPU starts without a thread
+    // wake-up (tens of microseconds), at the cost of one busy core.
     void spin_for_work(unsigned long long seen) {
         const auto until = Clock::now() + std::chrono::microseconds(kSpinMicros);
         while (events.load(std::memory_order_acquire) == seen && Clock::now() < until) {
@@ -422,6 +425,12 @@ struct Server::Impl {
             return 0;
         };
         const double t0 = now();
+        double first_wait = 0.0;  // longest submit-to-launch wait of a request's first chunk
+        for (const auto &a : take) {
+            if (a->next_seq == 0) {
+                first_wait = std::max(first_wait, t0 - a->submit_ms);
+            }
+        }
         rps_stats st{};
         const int rc = rps_sieve_many(ctx, reqs.data(), n, sink, &found, &st);
         const double wall = now() - t0;
@@ -435,9 +444,13 @@ struct Server::Impl {
         }
         if (options.progress) {
             std::fprintf(stderr, "serve: round %llu: %zu requests, %.4g sites, kernel %.2f ms, "
-                         "call %.2f ms, %zu active\n", round_no, n, sites, st.sieve_ms, wall,
-                         active.size());
+                         "call %.2f ms (plan %.3f, tables %.3f, sieve %.3f), %zu active, "
+                         "idle before %.3f ms, new request waited %.3f ms\n",
+                         round_no, n, sites, st.sieve_ms, wall, st.plan_ms + st.workspace_ms,
+                         st.basis_wall_ms, st.sieve_wall_ms, active.size(), t0 - last_round_end,
+                         first_wait);
         }
+        last_round_end = now();
         if (rc != RPS_OK) {
             const std::string error = std::string("GPU round failed: ") + rps_last_error(ctx);
             for (auto &a : active) {

@@ -78,6 +78,7 @@ def gpu(pkg):
 @pytest.fixture
 def fresh_gpu(gpu):
     gpu.set_option("variant", -1)
+    gpu.set_option("run_log2", -1)
     gpu.set_option("survivor_capacity", 1 << 24)
     gpu.set_option("batch_size", 65536)
     gpu.set_option("min_tiles", 2048)

@@ -53,10 +53,33 @@ def test_tuning_table_names_compiled_variants(pkg):
     import os
     names = pkg.variants()
     table = os.path.join(os.path.dirname(pkg.LIB_PATH), "..", "csrc", "tuning_table.inc")
-    listed = re.findall(r'\{[^,]+, "([^"]+)"\}', open(table).read())
-    assert listed and all(n in names for n in listed)
+    rows = re.findall(r'\{([^,]+), "([^"]+)", (\d+), (\d+), "([^"]+)", (\d+)\}',
+                      open(table).read())
+    assert rows and all(r[1] in names and r[4] in names for r in rows)
     for words in (1, 769, 2048, 10 ** 5, 3 * 10 ** 10):
         assert 0 <= pkg.tuned_variant(words) < len(names)
+        for many in (False, True):
+            v, run_log2, batch = pkg.tuned(words, many)
+            assert 0 <= v < len(names) and 10 <= run_log2 <= 24
+            assert batch == 0 or (not many and 1 <= batch <= 1 << 20)
+    # The table's rows are what the lookup returns, bucket by bucket.
+    for upper, name, run_log2, batch, many_name, many_run in rows:
+        words = (1 << 40) if upper.startswith("~") else int(upper.rstrip("ULL"))
+        assert pkg.tuned(words, False) == (names.index(name), int(run_log2), int(batch))
+        assert pkg.tuned(words, True) == (names.index(many_name), int(many_run), 0)
+
+
+def test_variant_info(pkg):
+    for i, name in enumerate(pkg.variants()):
+        info = pkg.variant_info(i)
+        m = re.fullmatch(r"([qw]_)?u(\d)_f(\d+)_w(\d+)", name)
+        assert info["words_per_lane"] == int(m.group(2))
+        assert info["dense_primes"] == int(m.group(3))
+        assert info["waves"] == int(m.group(4))
+        assert info["wide"] == (1 if name.startswith("w_") else 0)
+        assert 10 <= info["run_log2"] <= 24
+    with pytest.raises(ValueError):
+        pkg.variant_info(len(pkg.variants()))
 
 
 def test_library_targets_gfx950(pkg):

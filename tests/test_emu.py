@@ -50,7 +50,8 @@ def variants(ctx):
 
 
 def defaults(ctx):
-    for name, value in (("variant", -1), ("survivor_capacity", 1 << 24), ("batch_size", 65536),
+    for name, value in (("variant", -1), ("run_log2", -1), ("survivor_capacity", 1 << 24),
+                        ("batch_size", 65536),
                         ("min_tiles", 2048), ("direct_words", 768), ("primitive_only", 0)):
         ctx.set_option(name, value)
     return ctx
@@ -235,3 +236,21 @@ def test_mode_coverage_trace_at_full_line_shapes(workloads, tmp_path):
             assert proc.stderr.count("serve: round") >= 3, proc.stderr
         results[mode] = test_serve.parse_results(out)
     test_serve.check(reqs + small, results, oracle_limit=10_000)
+
+
+def test_run_lengths_match_oracle(emu, oracle_lib):
+    # Whole-line runs of other lengths than a variant's own (the "run_log2"
+    # option the autotuner sweeps), plain and many-curve paths.
+    coeffs, _ = curves.partially_dense(random.Random(17), 7, 5)
+    ref = oracle_lib.sieve(coeffs, 30000, 11, 60, method="words")
+    for name in ("q_u1_f12_w8", "w_u2_f14_w16"):
+        for run_log2 in (10, 13):
+            defaults(emu).set_option("variant", variants(emu).index(name))
+            emu.set_option("run_log2", run_log2)
+            emu.set_option("min_tiles", 1)
+            assert same(emu.sieve(coeffs, 30000, 11, 60), ref), (name, run_log2)
+            results, stats = emu.sieve_many([{"coefficients": coeffs, "bound": 30000,
+                                              "first": 11, "last": 60}])
+            num, den = results[0]
+            assert np.array_equal(num, ref[2]) and np.array_equal(den, ref[3]), (name, run_log2)
+    defaults(emu)

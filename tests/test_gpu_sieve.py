@@ -202,6 +202,24 @@ def test_runs_of_lines(fresh_gpu, oracle_lib, pkg, variant):
 
 
 @pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
+def test_run_lengths(fresh_gpu, oracle_lib, pkg, variant):
+    # Run lengths other than the variant's own (the "run_log2" option the
+    # autotuner sweeps): 2,501-word lines in runs of 1, 3 and 13 lines, on
+    # the plain and the many-curve path.
+    coeffs, _ = curves.partially_dense(random.Random(190 + variant), 7, 6)
+    ref = oracle_lib.sieve(coeffs, 80000, 7, 140, method="words")
+    fresh_gpu.set_option("variant", variant)
+    fresh_gpu.set_option("min_tiles", 1)
+    for run_log2 in (11, 13, 15):
+        announce(case="run-length", variant=pkg.variants()[variant], run_log2=run_log2)
+        fresh_gpu.set_option("run_log2", run_log2)
+        assert_same(fresh_gpu.sieve(coeffs, 80000, 7, 140), ref)
+        results, _ = fresh_gpu.sieve_many([{"coefficients": coeffs, "bound": 80000,
+                                            "first": 7, "last": 140}])
+        assert np.array_equal(results[0][0], ref[2]) and np.array_equal(results[0][1], ref[3])
+
+
+@pytest.mark.parametrize("variant", VARIANTS, ids=VARIANT_IDS)
 def test_runs_resume_exactly_once(fresh_gpu, oracle_lib, pkg, variant):
     # Runs of 104 lines of 1,251 words with about a quarter of all sites
     # surviving: waves stop mid-run on a full 16,384-survivor buffer and
