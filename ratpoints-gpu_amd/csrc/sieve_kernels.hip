@@ -307,6 +307,7 @@ struct RowTile {
     // rounded up to whole waves.
     static constexpr int kEntries = WIDE ? (slot_elems(0) + 64 + 63) / 64 * 64 : kSlot;
     static constexpr int kFast = FAST;
+    static constexpr bool kWide = WIDE;
     // LDS layout, in 64-bit words: rows, the dense primes' residue strings,
     // then two sets (by line parity) of the per-line arrays: six arrays of
     // 32 uint32 (p, (-A) mod p, b mod p, candidate index, ts mod p, magic)
@@ -379,6 +380,71 @@ struct RowTile {
         }
         if (kThreads < kEntries && e0 < static_cast<uint32_t>(kEntries - kThreads)) {
             build_entries(pool, cext, P, C, M, e0 + kThreads);
+        }
+    }
+
+    // Wide rows: the build as (row, 64-entry column) units dealt to the
+    // waves in turn, so that every wave of the workgroup takes part (with
+    // one thread per entry only the first kEntries threads had work, and
+    // the line waited for them: 10 of 16 waves building 14 entries each);
+    // a unit's row is wave-uniform.  Units are taken two at a time, their
+    // string reads in flight together.
+    static constexpr int kCols = kEntries / 64;
+    static constexpr int kUnits = FAST * kCols;
+    static constexpr int kUnitsPerWave = (kUnits + WAVES - 1) / WAVES;
+    static constexpr int kUnitGroup = 2;
+
+    __device__ __forceinline__ static void build_rows_dealt(uint64_t *pool, int lane, int wave,
+                                                            int par) {
+        const uint32_t *s_p = small(pool, par);
+        const uint32_t *s_c = s_p + kSelected;
+        const uint32_t *s_magic = s_p + 5 * kSelected;
+        const char *cext = reinterpret_cast<const char *>(pool + kCext);
+#pragma unroll
+        for (int j0 = 0; j0 < kUnitsPerWave; j0 += kUnitGroup) {
+            uint32_t w_lo[kUnitGroup], w_hi[kUnitGroup], e[kUnitGroup], elems[kUnitGroup];
+            int base[kUnitGroup];
+            bool live[kUnitGroup];
+#pragma unroll
+            for (int g = 0; g < kUnitGroup; ++g) {
+                const int unit = wave + WAVES * (j0 + g);
+                const int s = unit / kCols;
+                const int col = unit - s * kCols;
+                // Row s's base and element count, wave-uniform.
+                int b = 0, n = 0;
+#pragma unroll
+                for (int k = 0; k < FAST; ++k) {
+                    b += k < s ? 2 * slot_elems(k) : 0;
+                    n = k == s ? slot_elems(k) : n;
+                }
+                base[g] = b;
+                elems[g] = static_cast<uint32_t>(n);
+                live[g] = j0 + g < kUnitsPerWave && unit < kUnits && 64 * col < n + 64;
+                const int sr = live[g] ? s : 0;
+                const uint32_t pp = uniform(s_p[sr]);
+                const uint32_t cc = uniform(s_c[sr]);
+                const uint32_t mg = uniform(s_magic[sr]);
+                e[g] = 64u * static_cast<uint32_t>(col) + static_cast<uint32_t>(lane);
+                const uint32_t o = live[g] ? mod_small(cc + 64u * e[g], pp, mg) : 0u;
+                const uint32_t *d = reinterpret_cast<const uint32_t *>(cext + sr * 80) + (o >> 5);
+                const uint32_t d0 = d[0];
+                const uint32_t d1 = d[1];
+                const uint32_t d2 = d[2];
+                w_lo[g] = __builtin_amdgcn_alignbit(d1, d0, o & 31);
+                w_hi[g] = __builtin_amdgcn_alignbit(d2, d1, o & 31);
+            }
+#pragma unroll
+            for (int g = 0; g < kUnitGroup; ++g) {
+                if (live[g]) {
+                    const uint64_t w = (static_cast<uint64_t>(w_hi[g]) << 32) | w_lo[g];
+                    if (e[g] < elems[g]) {
+                        pool[base[g] + 2 * e[g]] = w;
+                    }
+                    if (e[g] >= 64u && e[g] < elems[g] + 64u) {
+                        pool[base[g] + 2 * (e[g] - 64) + 1] = w;
+                    }
+                }
+            }
         }
     }
 
@@ -1192,7 +1258,11 @@ struct RowRun {
         if (Base::kPlanned && uniform(Base::small(pool, r & 1)[Base::kDefaultFlag])) {
             Base::build_planned(pool, plan, tid, lane, wave);
         } else {
-            Base::build_rows(pool, tid, r & 1);
+            if constexpr (Base::kWide) {
+                Base::build_rows_dealt(pool, lane, wave, r & 1);
+            } else {
+                Base::build_rows(pool, tid, r & 1);
+            }
         }
         if (wave == 0 && r + 1 < nr) {
             select(pool,
