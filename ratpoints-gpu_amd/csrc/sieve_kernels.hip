@@ -23,8 +23,6 @@
 #include <cstdint>
 #include <mutex>
 
-#include <gfx950_asm.hpp>
-
 #include "sieve_kernels.hpp"
 #include "sieve_layout.hpp"
 
@@ -593,90 +591,84 @@ struct RowTile {
         }
     }
 
-    // Row offsets of a wave: lane s < FAST holds dense prime s and the
-    // offset W mod p_s of the wave's next iteration; the other lanes hold 0
-    // and p = kStride, so their offset stays 0.  The first kScalarRows rows
-    // of the wide tile also keep theirs in SGPRs, advanced by three scalar
-    // instructions each: that balances the scalar unit (idle with every
-    // offset in the VGPR) against the vector pipes (busiest with it).
-    // Measured on 156,251-word lines: 0 rows 53.8-53.9, 3 rows 54.8-55.0,
-    // 4-6 rows 55.1-55.3, 7 rows 54.6, all 14 rows 51.6-51.8 Tsite/s; the
-    // narrow tiles gain nothing from it.
-    static constexpr int kScalarRows = WIDE ? (FAST < 5 ? FAST : 5) : 0;
+    // Row offsets of a wave, in bytes (entries of 8 bytes, wide elements of
+    // 16): lane l holds dense row s = l mod 16, when s < FAST, with its
+    // prime and the offset W mod p_s of the wave's next iteration, so each
+    // 16-lane row of the wave holds every offset.  A DPP row broadcast
+    // (row_newbcast:s) hands offset s to all lanes inside the address add,
+    // one vector instruction per dense read; with v_readlane into an SGPR
+    // and an add it took two, and with offsets kept in SGPRs three scalar
+    // instructions per row and iteration.  Lanes with s >= FAST hold 0 and
+    // a modulus of one stride, so their offset stays 0.
+    static constexpr uint32_t kEntryBytes = WIDE ? 16u : 8u;
+    static_assert(FAST <= 16, "one dense row per lane of a 16-lane DPP row");
     struct Offsets {
         uint32_t off;
         uint32_t p;
-        uint32_t soff[kScalarRows > 0 ? kScalarRows : 1];
-        uint32_t sp[kScalarRows > 0 ? kScalarRows : 1];
     };
 
-    // Lane s < FAST gets prime s and the offset of word `rel` (from the tile
-    // start) in its row.
+    // Lanes of row s get prime s and the byte offset of word `rel` (from
+    // the tile start) in its row.
     __device__ __forceinline__ static Offsets row_offsets(const uint32_t *s_p,
                                                           const uint32_t *s_base,
                                                           const uint32_t *s_magic, uint32_t rel,
                                                           int lane) {
         Offsets o{};
         o.off = 0u;
-        o.p = static_cast<uint32_t>(kStride);
-        if (lane < FAST) {
-            const uint32_t p = s_p[lane];
-            const uint32_t v = mod_small(rel, p, s_magic[lane]) + s_base[lane];
-            o.off = v >= p ? v - p : v;
-            o.p = p;
-        }
-#pragma unroll
-        for (int s = 0; s < kScalarRows; ++s) {
-            o.soff[s] = __builtin_amdgcn_readlane(o.off, s);
-            o.sp[s] = __builtin_amdgcn_readlane(o.p, s);
+        o.p = static_cast<uint32_t>(kStride) * kEntryBytes;
+        const int s = lane & 15;
+        if (s < FAST) {
+            const uint32_t p = s_p[s];
+            const uint32_t v = mod_small(rel, p, s_magic[s]) + s_base[s];
+            o.off = (v >= p ? v - p : v) * kEntryBytes;
+            o.p = p * kEntryBytes;
         }
         return o;
     }
 
-    // Dense stage of one iteration: m[u] = AND over the FAST rows of entry
-    // off_s + lane + 64 u, then every offset advances one stride (mod p).
-    // Rows are combined two at a time on 32-bit halves with the three-input
-    // AND (v_bitop3_b32), one instruction per half for two rows.  The
-    // offsets live in one VGPR (lane s = row s): read back with v_readlane
-    // and advanced by three vector instructions for all rows, instead of
-    // three scalar instructions per row, which made the scalar unit the
-    // limit of the loop; the first kScalarRows rows use their SGPR copies.
-    __device__ __forceinline__ static void dense(const uint64_t *pool, Offsets &o, int lane,
-                                                 uint64_t (&m)[U]) {
-        uint32_t off[FAST];
-#pragma unroll
-        for (int s = 0; s < FAST; ++s) {
-            off[s] = s < kScalarRows ? o.soff[s < kScalarRows ? s : 0]
-                                     : __builtin_amdgcn_readlane(o.off, s);
-        }
-        // All loads first, so that each wave keeps FAST reads in flight.
-        uint64_t x[FAST][U];
-#pragma unroll
-        for (int s = 0; s < FAST; ++s) {
+    // Byte address of this lane's element in dense row S: the row's offset,
+    // broadcast from lane S of the lane's 16-lane row, plus the lane's own
+    // element.  The DPP move folds into the add (v_add_u32_dpp).
+    template <int S>
+    __device__ __forceinline__ static uint32_t row_address(uint32_t off, uint32_t lane_bytes) {
+        const uint32_t o = static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(
+            static_cast<int>(off), 0x150 + S /* row_newbcast:S */, 0xf, 0xf, true));
+        return static_cast<uint32_t>(row_base(S)) * 8u + o + lane_bytes;
+    }
+
+    // Loads dense rows S.. of one iteration into x.
+    template <int S>
+    __device__ __forceinline__ static void load_rows(const uint64_t *pool, uint32_t off,
+                                                     uint32_t lane_bytes, uint64_t (&x)[FAST][U]) {
+        if constexpr (S < FAST) {
+            const char *at = reinterpret_cast<const char *>(pool) + row_address<S>(off, lane_bytes);
             if (WIDE) {
-                const u64x2 v = reinterpret_cast<const u64x2 *>(pool + row_base(s))[off[s] + lane];
-                x[s][0] = v.x;
-                x[s][U - 1] = v.y;
+                const u64x2 v = *reinterpret_cast<const u64x2 *>(at);
+                x[S][0] = v.x;
+                x[S][U - 1] = v.y;
             } else {
-                const uint64_t *r = pool + row_base(s) + off[s] + lane;
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    x[s][u] = r[64 * u];
+                    x[S][u] = reinterpret_cast<const uint64_t *>(at)[64 * u];
                 }
             }
+            load_rows<S + 1>(pool, off, lane_bytes, x);
         }
+    }
+
+    // Dense stage of one iteration: m[u] = AND over the FAST rows of entry
+    // off_s + lane + 64 u, then every offset advances one stride (mod p,
+    // three vector instructions for all rows).  Rows are combined two at a
+    // time on 32-bit halves with the three-input AND (v_bitop3_b32), one
+    // instruction per half for two rows.
+    __device__ __forceinline__ static void dense(const uint64_t *pool, Offsets &o, int lane,
+                                                 uint64_t (&m)[U]) {
+        // All loads first, so that each wave keeps FAST reads in flight.
+        uint64_t x[FAST][U];
+        load_rows<0>(pool, o.off, static_cast<uint32_t>(lane) * kEntryBytes, x);
         {
-            const uint32_t next = o.off + kStride;
+            const uint32_t next = o.off + static_cast<uint32_t>(kStride) * kEntryBytes;
             o.off = next >= o.p ? next - o.p : next;
-        }
-        // The scalar copies advance while the reads are in flight
-        // (keep_in_sgpr pins the new values in SGPRs here, ahead of the AND
-        // and the queue branch).
-#pragma unroll
-        for (int s = 0; s < kScalarRows; ++s) {
-            const uint32_t next = o.soff[s] + kStride;
-            o.soff[s] = next >= o.sp[s] ? next - o.sp[s] : next;
-            keep_in_sgpr(o.soff[s]);
         }
         uint32_t lo[U], hi[U];
 #pragma unroll

@@ -49,6 +49,7 @@ enum Op : int {
     kShflXor,
     kReadLane,
     kReadFirstLane,
+    kRowBcast,
     kWaveSync,
     kBarrier,
     kExited,
@@ -63,6 +64,8 @@ const Idx &block_idx();
 const Idx &block_dim();
 const Idx &grid_dim();
 int lane_id();
+// Lane n of row_newbcast:n (dpp_ctrl 0x150 + n).
+int row_bcast_lane(int dpp_ctrl);
 // A cross-lane operation: blocks the lane until its wave resolves it.
 uint64_t wave_op(Op op, uint64_t value, int arg, int line);
 void barrier(int line);
@@ -134,6 +137,12 @@ inline int __ffsll(long long x) { return __builtin_ffsll(x); }
     (static_cast<uint32_t>(::emu::wave_op(::emu::kReadFirstLane, static_cast<uint32_t>(v), 0, __LINE__)))
 #define __builtin_amdgcn_readlane(v, l) \
     (static_cast<uint32_t>(::emu::wave_op(::emu::kReadLane, static_cast<uint32_t>(v), (l), __LINE__)))
+// DPP row broadcast, the one DPP control the kernels use (row_newbcast:n,
+// dpp_ctrl 0x150 + n): every lane takes lane n of its 16-lane row.  Any
+// other control ends the process.
+#define __builtin_amdgcn_mov_dpp(v, ctrl, row_mask, bank_mask, bound_ctrl) \
+    (static_cast<int>(::emu::wave_op(::emu::kRowBcast, static_cast<uint32_t>(v), \
+                                     ::emu::row_bcast_lane(ctrl), __LINE__)))
 #define __builtin_amdgcn_wave_barrier() ((void)::emu::wave_op(::emu::kWaveSync, 0, 0, __LINE__))
 #define __builtin_amdgcn_fence(order, scope) ((void)0)
 #define __builtin_amdgcn_s_sleep(n) ((void)0)

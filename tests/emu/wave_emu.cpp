@@ -114,6 +114,7 @@ const char *op_name(int op) {
     case kShflXor: return "shfl_xor";
     case kReadLane: return "readlane";
     case kReadFirstLane: return "readfirstlane";
+    case kRowBcast: return "row_newbcast";
     case kWaveSync: return "wave_barrier";
     case kBarrier: return "__syncthreads";
     case kExited: return "returned";
@@ -319,6 +320,17 @@ void resolve(int w, int op) {
         case kReadFirstLane:
             r = value[first];
             break;
+        case kRowBcast: {
+            if (L[l].arg != L[first].arg) {
+                fatal("row_newbcast with a lane index that is not wave-uniform");
+            }
+            const int src = (l & ~15) + L[l].arg;
+            if (!live(src)) {
+                fatal("row_newbcast from a lane that is inactive or has returned");
+            }
+            r = value[src];
+            break;
+        }
         case kWaveSync:
             break;
         default:
@@ -518,6 +530,13 @@ const Idx &block_idx() { return g.block; }
 const Idx &block_dim() { return g.bdim; }
 const Idx &grid_dim() { return g.gdim; }
 int lane_id() { return g.current->lane; }
+
+int row_bcast_lane(int dpp_ctrl) {
+    if (dpp_ctrl < 0x150 || dpp_ctrl > 0x15f) {
+        fatal("DPP control other than row_newbcast");
+    }
+    return dpp_ctrl - 0x150;
+}
 
 uint64_t wave_op(Op op, uint64_t value, int arg, int line) {
     Lane *l = g.current;
