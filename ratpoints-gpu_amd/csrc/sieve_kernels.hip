@@ -502,6 +502,13 @@ struct RowTile {
     static constexpr int kTailCols = kEntries > kThreads ? (kEntries - kThreads) / 64 : 0;
     static constexpr int kTailUnits = FAST * kTailCols;
     static constexpr int kTailPerWave = (kTailUnits + WAVES - 1) / WAVES;
+    // Tail unit of wave `wave` in its i-th turn.  The units start at wave
+    // 1, so the waves with the most of them are not wave 0 and the last
+    // wave, which also select the primes of the next two lines during the
+    // build.
+    __device__ __forceinline__ static int tail_unit(int wave, int i) {
+        return (wave + WAVES - 1) % WAVES + WAVES * i;
+    }
     struct Plan {
         uint32_t main[kPlanned ? FAST : 1];
         uint32_t tail[kPlanned && kTailPerWave > 0 ? kTailPerWave : 1];
@@ -529,7 +536,7 @@ struct RowTile {
             }
 #pragma unroll
             for (int i = 0; i < kTailPerWave; ++i) {
-                const int j = wave + WAVES * i;
+                const int j = tail_unit(wave, i);
                 if (j < kTailUnits) {
                     const int s = j % FAST;
                     const int k = kCandidates - 1 - s;
@@ -555,25 +562,26 @@ struct RowTile {
     }
 
     // Dense rows of a line with the default selection, from the plan.
+    static constexpr int kBuildGroup = 6;
     __device__ __forceinline__ static void build_planned(uint64_t *pool, const Plan &pl, int tid,
                                                          int lane, int wave) {
         if (!kPlanned) {
             return;
         }
         if (tid < kEntries) {
-            // Four rows' reads in flight at a time keeps the register peak
-            // of the set-up below the sweep's.
+            // kBuildGroup rows' reads in flight at a time keeps the register
+            // peak of the set-up below the sweep's.
 #pragma unroll
-            for (int s0 = 0; s0 < FAST; s0 += 4) {
-                uint64_t w[4];
+            for (int s0 = 0; s0 < FAST; s0 += kBuildGroup) {
+                uint64_t w[kBuildGroup];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < kBuildGroup; ++j) {
                     if (s0 + j < FAST) {
                         w[j] = plan_window(pool, pl.main[s0 + j]);
                     }
                 }
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < kBuildGroup; ++j) {
                     if (s0 + j < FAST) {
                         pool[row_base(s0 + j) + tid] = w[j];
                     }
@@ -582,7 +590,7 @@ struct RowTile {
         }
 #pragma unroll
         for (int i = 0; i < kTailPerWave; ++i) {
-            const int j = wave + WAVES * i;
+            const int j = tail_unit(wave, i);
             if (j < kTailUnits) {
                 const int s = j % FAST;
                 const int e = kThreads + 64 * (j / FAST) + lane;
@@ -957,6 +965,11 @@ struct RowTileQ {
             }
             it = 1;
         }
+        // The iteration count is wave-uniform; saying so keeps it (and the
+        // loop control below) in SGPRs, where the select above would
+        // otherwise have left it in a VGPR and cost vector instructions
+        // every iteration.
+        it = uniform(it);
         // Wave priority falls with the wave's progress through its range:
         // 3 in its first quarter down to 0 in its last.  The waves of a
         // workgroup sweep equal ranges, but the SIMDs issue the oldest waves
@@ -1032,7 +1045,8 @@ struct RowTileQ {
         for (int u = 0; u < U; ++u) {
             any |= m[u];
         }
-        if (!__ballot(any != 0)) {
+        const uint64_t any_bal = __ballot(any != 0);
+        if (!any_bal) {
             return true;
         }
         uint64_t *qm = queue_masks(pool, wave);
@@ -1040,7 +1054,8 @@ struct RowTileQ {
         const uint32_t rel = (rel_w + lane) | ln.tag;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const uint64_t bal = __ballot(m[u] != 0);
+            // With one word per lane that ballot is the word's own.
+            const uint64_t bal = U == 1 ? any_bal : __ballot(m[u] != 0);
             if (!bal) {
                 continue;
             }
