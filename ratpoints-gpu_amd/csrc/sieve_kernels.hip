@@ -1950,6 +1950,8 @@ const VariantInfo kVariants[] = {
     RPS_VARIANT_Q(2, 12, 16),
     RPS_VARIANT_W(14, 16),
     RPS_VARIANT_W(13, 16),
+    RPS_VARIANT_Q(1, 11, 8),
+    RPS_VARIANT_Q(1, 13, 8),
 };
 
 #undef RPS_VARIANT_Q

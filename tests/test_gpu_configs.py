@@ -15,6 +15,7 @@ and comes out of the exact stage.
 The oracle runs on 16 host threads (the box's CPU share).
 """
 
+import math
 import os
 import subprocess
 
@@ -109,3 +110,75 @@ def test_batched_4096_curves_match_oracle(fresh_gpu, oracle_lib, workloads):
     assert not bad, bad[:10]
     print(f"whole boxes of {len(whole)} curves: identical", flush=True)
     assert len(whole) >= 700
+
+
+DOUBLE = os.path.join(ROOT, "oracle", "_build", "ratpoints_gpu_oracle")
+
+
+def _merged(intervals):
+    """The request's closed x-intervals sorted and merged: the CLI takes -l/-u
+    pairs in increasing order (src/command_line.cpp:191-259)."""
+    out = []
+    for lo, hi in sorted(intervals):
+        if out and lo <= out[-1][1]:
+            out[-1] = (out[-1][0], max(out[-1][1], hi))
+        else:
+            out.append((lo, hi))
+    return out
+
+
+def _mode_args(r):
+    args = [" ".join(map(str, r["coefficients"])), str(r["bound"]), "-dl", str(r["first"]),
+            "-du", str(r["last"]), "-y"]
+    if r["square"]:
+        args.append("--square-denominators")
+    for lo, hi in _merged(r["intervals"]):
+        args += ["-l", "%.9f" % float(lo), "-u", "%.9f" % float(hi)]
+    return args
+
+
+def test_mode_coverage_curves_one_search_each(fresh_gpu, oracle_lib, workloads):
+    """Mode-coverage shapes, one search at a time on the plain path:
+    square-denominator cubics and quartics (64-128-bit coefficients, b = k^2)
+    and degree 50-100 curves with 1,024-bit coefficients, each with 1-3
+    x-intervals, points at infinity on and ordinates suppressed (-y).  Per
+    request: the survivor set bit-identical to the oracle's; the CLI's stdout
+    byte-identical to that of the same CLI over the oracle's C ABI double
+    (oracle/_build/ratpoints_gpu_oracle); every planted point that lies in an
+    interval printed.  Boxes: k <= 300 at |a| <= 20,000 and height 400, plus
+    survivor sets for k up to 46,340 (b just below 2^31) at |a| <= 60, where
+    the CLI, like the reference's, caps k at the height."""
+    from fractions import Fraction
+    if not os.path.exists(DOUBLE):
+        pytest.skip("oracle CLI double not built")
+    reqs = workloads.mode_coverage_requests(seed=4, count=24, square_k=300,
+                                            square_bound=20_000, high_height=400)
+    reqs += workloads.mode_coverage_requests(seed=5, count=4, square_k=46_340,
+                                             square_bound=60, high_height=30)
+    printed = 0
+    for r in reqs:
+        announce(case="mode-coverage", degree=r["degree"], square=r["square"],
+                 A=r["bound"], last=r["last"])
+        coeffs = r["coefficients"]
+        res = fresh_gpu.sieve(coeffs, r["bound"], r["first"], r["last"], square=r["square"])
+        count, _, on, od = oracle_lib.sieve(coeffs, r["bound"], r["first"], r["last"],
+                                            square=r["square"], method="words",
+                                            threads=THREADS)
+        assert res.survivor_count == count
+        assert np.array_equal(res.numerators, on) and np.array_equal(res.denominators, od)
+        args = _mode_args(r)
+        gpu = subprocess.run([CLI, *args], capture_output=True, text=True, timeout=300)
+        ref = subprocess.run([DOUBLE, *args], capture_output=True, text=True, timeout=300)
+        assert gpu.returncode == 0 and ref.returncode == 0, (gpu.stderr, ref.stderr)
+        assert gpu.stdout == ref.stdout
+        lines = set(gpu.stdout.splitlines())
+        # The CLI caps the denominator bound at the height
+        # (src/command_line.cpp:353-356), k in square mode.
+        cap = min(r["last"], r["bound"])
+        for a, b in r["points"]:
+            k = math.isqrt(b) if r["square"] else b
+            x = Fraction(a, b)
+            if k <= cap and any(lo <= x <= hi for lo, hi in _merged(r["intervals"])):
+                assert f"({a} : {b})" in lines, (r["id"], a, b)
+        printed += len(lines)
+    assert printed > 0
