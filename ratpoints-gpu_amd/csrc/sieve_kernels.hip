@@ -287,11 +287,12 @@ struct RowTile {
     static constexpr int kWaves = WAVES;
     static constexpr int kThreads = WAVES * 64;
     static constexpr int kStride = 64 * U;
-    // Row elements per slot: >= p + 64 U - 1 entries (kSlot, every row), or
-    // p + 63 wide elements, sized per row by the largest prime it can hold.
+    // Row elements per slot, sized per row by the largest prime P_s it can
+    // hold: P_s + 64 U entries (>= p + 64 U - 1, the most a sweep reads), or
+    // P_s + 64 wide elements.  kSlot bounds them all.
     static constexpr int kSlot = WIDE ? kMaxPrime + 64 : kMaxPrime + 64 * U;
     __host__ __device__ static constexpr int slot_elems(int s) {
-        return WIDE ? largest_prime_below_512(s) + 64 : kSlot;
+        return largest_prime_below_512(s) + (WIDE ? 64 : 64 * U);
     }
     // First 64-bit word of row s.
     __host__ __device__ static constexpr int row_base(int s) {
@@ -359,11 +360,10 @@ struct RowTile {
         }
         // Thread t computes entry t of every row, and entry t + kThreads
         // while that is below kEntries (a wave-uniform test, as kEntries and
-        // kThreads are multiples of 64).  Narrow rows: entries from p +
-        // kStride - 1 to the end of the slot are padding the sweep never
-        // reads, so they are written without a test.  Wide rows: each entry
-        // goes into the two elements that hold it, when row s has them (a
-        // per-row bound).  All string reads of a thread are independent and
+        // kThreads are multiples of 64).  Narrow rows: entry e goes into row
+        // s when the row's slot has it (a per-row bound).  Wide rows: each
+        // entry goes into the two elements that hold it, when row s has them
+        // (a per-row bound).  All string reads of a thread are independent and
         // can be in flight together; the window at bit o is two funnel
         // shifts of three 32-bit string words.
         static_assert(kSlot % 64 == 0 && kEntries % 64 == 0 && kThreads % 64 == 0,
@@ -479,7 +479,7 @@ struct RowTile {
                 if (e >= 64u && e < n + 64u) {
                     pool[row_base(s) + 2 * (e - 64) + 1] = w;
                 }
-            } else {
+            } else if (e < static_cast<uint32_t>(slot_elems(s))) {
                 pool[row_base(s) + e] = w;
             }
         }
@@ -582,7 +582,7 @@ struct RowTile {
                 }
 #pragma unroll
                 for (int j = 0; j < kBuildGroup; ++j) {
-                    if (s0 + j < FAST) {
+                    if (s0 + j < FAST && tid < slot_elems(s0 + j)) {
                         pool[row_base(s0 + j) + tid] = w[j];
                     }
                 }
@@ -594,7 +594,16 @@ struct RowTile {
             if (j < kTailUnits) {
                 const int s = j % FAST;
                 const int e = kThreads + 64 * (j / FAST) + lane;
-                pool[s * kSlot + e] = plan_window(pool, pl.tail[i]);
+                // Row s's base and size (s is wave-uniform).
+                int base = 0, size = 0;
+#pragma unroll
+                for (int k = 0; k < FAST; ++k) {
+                    base += k < s ? slot_elems(k) : 0;
+                    size = k == s ? slot_elems(k) : size;
+                }
+                if (e < size) {
+                    pool[base + e] = plan_window(pool, pl.tail[i]);
+                }
             }
         }
     }
@@ -1952,6 +1961,7 @@ const VariantInfo kVariants[] = {
     RPS_VARIANT_W(13, 16),
     RPS_VARIANT_Q(1, 11, 8),
     RPS_VARIANT_Q(1, 13, 8),
+    RPS_VARIANT_Q(1, 14, 8),
 };
 
 #undef RPS_VARIANT_Q
