@@ -1786,27 +1786,39 @@ void add_stats(rps_stats &into, const rps_stats &s) {
 // distinct curve on the device (1.33 MB each, 10.9 GB for this many).
 constexpr size_t kMaxCurvesPerLaunch = 8192;
 
-// With the variant left to the tuning table, requests whose lines want a
-// one-workgroup-per-CU (wide-row) tile run in a launch of their own, and
-// the others share a launch of the tuning table's choice for the largest of
-// them: a launch has one LDS layout, and the wide one would cost the short
-// lines half their occupancy.  Each group is cut into launches of at most
-// kMaxCurvesPerLaunch requests.
+// Sites below which a group of requests wanting one variant does not get a
+// launch of its own (about 5 ms of kernel time on mid-length lines).
+constexpr double kOwnLaunchSites = 1.5e11;
+
+// With the variant left to the tuning table, requests are grouped by the
+// many-curve column's choice for their line length, and each group runs in
+// launches of its own (a launch has one LDS layout and one tile shape: the
+// wide tile would cost short lines half their occupancy, and a narrow tile
+// with 14 dense rows builds more per line than short lines repay).  A group
+// with fewer than kOwnLaunchSites sites joins the group of its tile family
+// (wide or not) with the most sites, so that a small call (a serving round)
+// is not cut into many launches.  Each group is cut into launches of at
+// most kMaxCurvesPerLaunch requests.
 int sieve_many_split(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
                      rps_multi_sink sink, void *user, rps_stats *stats) {
     if (n_requests < 2 || !requests || !sink || n_requests > (size_t(1) << 20)) {
         return sieve_many_launch(ctx, requests, n_requests, sink, user, stats, -1);
     }
-    std::vector<int32_t> family[2];  // [0] two-workgroup tiles (or forced), [1] wide
+    const int n_variants = rps::variant_count();
+    std::vector<std::vector<int32_t>> by_variant(static_cast<size_t>(n_variants));
+    std::vector<double> group_sites(static_cast<size_t>(n_variants), 0.0);
     try {
         for (size_t r = 0; r < n_requests; ++r) {
             validate(&requests[r]);
-            int f = 0;
+            int v = 0;
             if (ctx->variant < 0) {
-                const Tuned t = tuned(line_geometry(&requests[r]).words, true);
-                f = rps::variant(t.variant)->wide ? 1 : 0;
+                const rps::LineDesc L = line_geometry(&requests[r]);
+                v = tuned(L.words, true).variant;
+                group_sites[static_cast<size_t>(v)] +=
+                    (2.0 * static_cast<double>(L.bound) + 1.0)
+                    * (static_cast<double>(requests[r].den_last) - requests[r].den_first + 1.0);
             }
-            family[f].push_back(static_cast<int32_t>(r));
+            by_variant[static_cast<size_t>(v)].push_back(static_cast<int32_t>(r));
         }
     } catch (const ApiError &e) {
         ctx->last_kernel_ms = 0.0;
@@ -1817,7 +1829,33 @@ int sieve_many_split(rps_ctx *ctx, const rps_request *requests, size_t n_request
         }
         return fail(ctx, e);
     }
-    if (n_requests <= kMaxCurvesPerLaunch && (family[0].empty() || family[1].empty())) {
+    // Small groups join the largest group of their tile family.
+    int lead[2] = {-1, -1};
+    for (int v = 0; v < n_variants; ++v) {
+        const int f = rps::variant(v)->wide ? 1 : 0;
+        if (!by_variant[static_cast<size_t>(v)].empty()
+            && (lead[f] < 0 || group_sites[static_cast<size_t>(v)] > group_sites[static_cast<size_t>(lead[f])])) {
+            lead[f] = v;
+        }
+    }
+    std::vector<std::vector<int32_t>> family;
+    for (int v = 0; v < n_variants; ++v) {
+        const int f = rps::variant(v)->wide ? 1 : 0;
+        if (!by_variant[static_cast<size_t>(v)].empty() && v != lead[f]
+            && group_sites[static_cast<size_t>(v)] < kOwnLaunchSites) {
+            std::vector<int32_t> &into = by_variant[static_cast<size_t>(lead[f])];
+            into.insert(into.end(), by_variant[static_cast<size_t>(v)].begin(),
+                        by_variant[static_cast<size_t>(v)].end());
+            by_variant[static_cast<size_t>(v)].clear();
+        }
+    }
+    for (std::vector<int32_t> &g : by_variant) {
+        if (!g.empty()) {
+            std::sort(g.begin(), g.end());
+            family.push_back(std::move(g));
+        }
+    }
+    if (n_requests <= kMaxCurvesPerLaunch && family.size() == 1) {
         return sieve_many_launch(ctx, requests, n_requests, sink, user, stats, -1);
     }
     std::vector<std::vector<int32_t>> groups;
