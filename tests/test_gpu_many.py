@@ -245,3 +245,36 @@ def test_primitive_only_filters_on_device(fresh_gpu, oracle_lib, pkg, seed):
                                                      method="words")
             pn, pd = primitive(on, od)
             assert np.array_equal(wn, pn) and np.array_equal(wd, pd)
+
+
+def test_many_variant_groups(fresh_gpu, oracle_lib, pkg):
+    # Requests whose line lengths the many-curve column gives different
+    # variants, each with more than the 1.5e11 sites that earn a launch of
+    # its own, plus a small request of each length that joins its family's
+    # largest group: every request's survivors equal the plain path's and,
+    # for the small ones, the oracle's; one launch per distinct variant at
+    # least.
+    rng = random.Random(61)
+    reqs, groups = [], set()
+    for words in (1024, 2048, 4096, 16384, 65536):
+        coeffs, _ = curves.partially_dense(rng, 12, rng.choice([3, 4, 6]))
+        bound = 32 * words - 32
+        lines = int(1.6e11 / (2 * bound + 1)) + 1
+        first = rng.randint(1, 1000)
+        reqs.append({"coefficients": coeffs, "bound": bound, "first": first,
+                     "last": first + lines - 1})
+        reqs.append({"coefficients": coeffs, "bound": bound, "first": 7, "last": 9})
+        groups.add(pkg.tuned(words, many=True)[0])
+    assert len(groups) >= 3
+    announce(case="many-variant-groups", n=len(reqs), groups=len(groups))
+    results, stats = fresh_gpu.sieve_many(reqs)
+    _, launches, _ = fresh_gpu.kernel_timing()
+    assert launches >= len(groups)
+    assert stats["requeued"] == 0
+    for r, (num, den) in zip(reqs, results):
+        pn, pd = plain(fresh_gpu, r)
+        assert np.array_equal(num, pn) and np.array_equal(den, pd)
+        if r["last"] - r["first"] < 10:
+            count, _, on, od = oracle_lib.sieve(r["coefficients"], r["bound"], r["first"],
+                                                r["last"], False, method="words")
+            assert np.array_equal(num, on) and np.array_equal(den, od)
