@@ -46,7 +46,7 @@ hipError_t launch_reset_many(DeviceState *state, unsigned long long *head, int *
 // most kDirectMaxWords words.
 constexpr int kDirectWaves = 8;
 constexpr int kDirectMaxDenominators = 16;
-constexpr unsigned kDirectMaxWords = 768;
+constexpr unsigned kDirectMaxWords = 2048;
 hipError_t launch_direct(const SieveParams &params, unsigned grid,
                          hipStream_t stream);
 

@@ -380,9 +380,24 @@ def test_curve_cache_switch(fresh_gpu, oracle_lib):
         assert_same(res, oracle_lib.sieve(coeffs, 4000, 1, 40, method="words"))
 
 
+@pytest.mark.parametrize("bound", [24577, 40000, 65503])
+def test_short_line_kernel_up_to_2048_words(fresh_gpu, oracle_lib, bound):
+    # The short-line kernel takes lines of up to 2,048 words when asked to
+    # (direct_words up to its limit): lines of 769-2,048 words against the
+    # oracle, and the option's bound.
+    coeffs, _ = curves.partially_dense(random.Random(bound), 6, 6)
+    first, last = 3, 40
+    announce(case="short-line-kernel", A=bound, b=f"{first}..{last}")
+    fresh_gpu.set_option("direct_words", 2048)
+    assert_same(fresh_gpu.sieve(coeffs, bound, first, last),
+                oracle_lib.sieve(coeffs, bound, first, last, method="words"))
+    with pytest.raises(Exception):
+        fresh_gpu.set_option("direct_words", 2049)
+
+
 @pytest.mark.parametrize("bound", [1, 31, 32, 100, 1000, 9000, 24543])
 def test_short_lines_both_modes_match_oracle(fresh_gpu, oracle_lib, bound):
-    # Lines of at most 768 words go to the short-line kernel; the same input
+    # Lines of at most 1,152 words go to the short-line kernel; the same input
     # through the row kernel (direct_words = 0) and the oracle must agree.
     coeffs, _ = curves.partially_dense(random.Random(bound), 6, 6)
     first, last = 100, 100 + max(40, 40000 // (2 * bound + 1))
