@@ -569,9 +569,8 @@ struct RowTile {
             return;
         }
         if (tid < kEntries) {
-            // kBuildGroup rows' reads in flight at a time (six measured
-            // 2-4% faster than four; all FAST at once, which fits the same
-            // registers, no faster than six).
+            // kBuildGroup rows' reads in flight at a time (all FAST at once
+            // fits the same registers but measured no faster than six).
 #pragma unroll
             for (int s0 = 0; s0 < FAST; s0 += kBuildGroup) {
                 uint64_t w[kBuildGroup];
