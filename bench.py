@@ -162,6 +162,22 @@ def add_digest(x, y):
     return x[0] + y[0], (x[1] + y[1]) & 0xFFFFFFFFFFFFFFFF
 
 
+def survivor_digests_by_request(req, num, den):
+    """survivor_digest of each request's survivors in one flush of a
+    many-curve call: (request indices, counts, sums), one pass over the
+    flush instead of one per request."""
+    import numpy as np
+    a = np.asarray(num, dtype=np.int64).view(np.uint64)
+    b = np.asarray(den, dtype=np.int32).astype(np.uint32).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        h = _mix64(a ^ _mix64(b + np.uint64(0x9E3779B97F4A7C15)))
+    order = np.argsort(req, kind="stable")
+    keys, starts, counts = np.unique(np.asarray(req)[order], return_index=True,
+                                     return_counts=True)
+    sums = np.add.reduceat(h[order], starts) if len(starts) else h[:0]
+    return keys, counts, sums
+
+
 def batched_compare(pkg, curves, label):
     """One rps_sieve_many call over `curves` against looping rps_sieve curve
     by curve on the same GPU; both must give the same survivor set per
@@ -172,12 +188,19 @@ def batched_compare(pkg, curves, label):
     digests = [(0, 0)] * len(curves)
 
     def many_sink(req, num, den):
-        for r in np.unique(req):
-            sel = req == r
-            digests[r] = add_digest(digests[r], survivor_digest(num[sel], den[sel]))
+        keys, counts, sums = survivor_digests_by_request(req, num, den)
+        for r, n, h in zip(keys.tolist(), counts.tolist(), sums.tolist()):
+            digests[r] = add_digest(digests[r], (n, h))
         return False
 
-    ctx.run_many(curves[:64], lambda q, n, d: False)  # warm tables and code
+    # Warm-up: room for every curve's residue table (allocated once per
+    # context, as a service would keep it) and the code paths, on 64 curves
+    # that are not in the timed batch (constant coefficient shifted), so the
+    # timed call builds every one of its tables.
+    ctx.set_option("table_slots", len(curves))
+    warm = [dict(c, coefficients=[str(int(c["coefficients"][0]) + 1)]
+                 + [str(x) for x in c["coefficients"][1:]]) for c in curves[:64]]
+    ctx.run_many(warm, lambda q, n, d: False)
     print(f"bench: batched {label}: {len(curves)} curves ({sites:.3e} sites)", file=sys.stderr,
           flush=True)
     t0 = time.perf_counter()

@@ -72,22 +72,19 @@ bool is_decimal(const char *text) {
     return true;
 }
 
-// Signed decimal text mod p, in [0, p), nine digits per step.
-int decimal_mod(const char *text, int p) {
-    const bool negative = *text == '-';
-    if (*text == '-' || *text == '+') {
-        ++text;
+// u^-1 mod p for 0 < u < p, p prime (extended Euclid).
+int mod_inverse(int u, int p) {
+    long long r0 = p, r1 = u, t0 = 0, t1 = 1;
+    while (r1 != 0) {
+        const long long q = r0 / r1;
+        const long long r2 = r0 - q * r1;
+        const long long t2 = t0 - q * t1;
+        r0 = r1;
+        r1 = r2;
+        t0 = t1;
+        t1 = t2;
     }
-    long long v = 0;
-    while (*text) {
-        long long chunk = 0, scale = 1;
-        for (int k = 0; k < 9 && *text; ++k, ++text) {
-            chunk = 10 * chunk + (*text - '0');
-            scale *= 10;
-        }
-        v = (v * (scale % p) + chunk) % p;
-    }
-    return static_cast<int>(negative && v ? p - v : v);
+    return static_cast<int>(t0 < 0 ? t0 + p : t0);
 }
 
 struct HipError {
@@ -284,6 +281,7 @@ struct rps_ctx {
     std::string error;
     std::vector<int> primes;
     int *d_primes = nullptr;
+    int *d_inverses = nullptr;  // [36][512] u^-1 mod p, in d_primes' allocation
     int *d_coef_res = nullptr;
     unsigned long long *d_chi = nullptr;
     unsigned long long *d_resid = nullptr;
@@ -506,12 +504,43 @@ rps::LineDesc line_geometry(const rps_request *req) {
     return L;
 }
 
+// Residues of every coefficient (signed decimal text) for every candidate,
+// in [0, p).  Each coefficient's text is cut into 9-digit chunks once, and
+// the chunks are folded per prime, v = (v * (10^digits mod p) + chunk) mod p,
+// so the digits are read once rather than once per prime.
 void coefficient_residues(const rps_request *req, const std::vector<int> &primes,
                           int n_coef, int *out /* [36][n_coef] */) {
-    for (int i = 0; i < rps::kCandidates; ++i) {
-        for (int k = 0; k < n_coef; ++k) {
-            out[static_cast<size_t>(i) * n_coef + k] = k < req->n_coefficients
-                ? decimal_mod(req->coefficients[k], primes[i]) : 0;
+    std::vector<long long> chunk, scale;
+    for (int k = 0; k < n_coef; ++k) {
+        if (k >= req->n_coefficients) {
+            for (int i = 0; i < rps::kCandidates; ++i) {
+                out[static_cast<size_t>(i) * n_coef + k] = 0;
+            }
+            continue;
+        }
+        const char *text = req->coefficients[k];
+        const bool negative = *text == '-';
+        if (*text == '-' || *text == '+') {
+            ++text;
+        }
+        chunk.clear();
+        scale.clear();
+        while (*text) {
+            long long c = 0, sc = 1;
+            for (int d = 0; d < 9 && *text; ++d, ++text) {
+                c = 10 * c + (*text - '0');
+                sc *= 10;
+            }
+            chunk.push_back(c);
+            scale.push_back(sc);
+        }
+        for (int i = 0; i < rps::kCandidates; ++i) {
+            const long long p = primes[i];
+            long long v = 0;
+            for (size_t j = 0; j < chunk.size(); ++j) {
+                v = (v * (scale[j] % p) + chunk[j]) % p;
+            }
+            out[static_cast<size_t>(i) * n_coef + k] = static_cast<int>(negative && v ? p - v : v);
         }
     }
 }
@@ -530,8 +559,8 @@ void prepare_curve(rps_ctx *ctx, const rps_request *req, rps_stats *st) {
                            hipMemcpyHostToDevice, ctx->compute));
     RPS_HIP(hipEventRecord(ctx->basis_start, ctx->compute));
     RPS_HIP(rps::launch_chi(ctx->d_coef_res, n, 1, ctx->d_primes, ctx->d_chi, ctx->compute));
-    RPS_HIP(rps::launch_resid(ctx->d_primes, ctx->d_chi, 1, nullptr, ctx->d_resid,
-                              ctx->compute));
+    RPS_HIP(rps::launch_resid(ctx->d_primes, ctx->d_inverses, ctx->d_chi, 1, nullptr,
+                              ctx->d_resid, ctx->compute));
     RPS_HIP(hipEventRecord(ctx->basis_end, ctx->compute));
     wait_event(ctx->basis_end, ctx->launch_timeout_ms, "residue-table build");
     float ms = 0.0f;
@@ -663,9 +692,21 @@ int rps_open(int device, rps_ctx **out) {
         RPS_HIP(hipStreamCreateWithFlags(&ctx->compute, hipStreamNonBlocking));
         RPS_HIP(hipStreamCreateWithFlags(&ctx->copy, hipStreamNonBlocking));
         ctx->primes = candidate_primes();
-        RPS_HIP(hipMalloc(&ctx->d_primes, rps::kCandidates * sizeof(int)));
-        RPS_HIP(hipMemcpy(ctx->d_primes, ctx->primes.data(),
-                          rps::kCandidates * sizeof(int), hipMemcpyHostToDevice));
+        // The candidates, then u^-1 mod p for each candidate and u (0 for u
+        // = 0 and u >= p), which every residue-table build reads.
+        std::vector<int> constants(static_cast<size_t>(rps::kCandidates) * (1 + rps::kMaxPrime), 0);
+        for (int i = 0; i < rps::kCandidates; ++i) {
+            const int p = ctx->primes[i];
+            constants[i] = p;
+            int *inv = constants.data() + rps::kCandidates + static_cast<size_t>(i) * rps::kMaxPrime;
+            for (int u = 1; u < p; ++u) {
+                inv[u] = mod_inverse(u, p);
+            }
+        }
+        RPS_HIP(hipMalloc(&ctx->d_primes, constants.size() * sizeof(int)));
+        RPS_HIP(hipMemcpy(ctx->d_primes, constants.data(), constants.size() * sizeof(int),
+                          hipMemcpyHostToDevice));
+        ctx->d_inverses = ctx->d_primes + rps::kCandidates;
         RPS_HIP(hipMalloc(&ctx->d_coef_res, static_cast<size_t>(rps::kCandidates)
                               * rps::kMaxCoefficients * sizeof(int)));
         RPS_HIP(hipMalloc(&ctx->d_chi, static_cast<size_t>(rps::kCandidates)
@@ -1192,7 +1233,8 @@ int rps_sieve_into(rps_ctx *ctx, const rps_request *req, int64_t *numerators,
         RPS_HIP(hipMemcpyAsync(negmod, ctx->h_negmod, rps::kCandidates * sizeof(int),
                                hipMemcpyHostToDevice, ctx->compute));
         RPS_HIP(rps::launch_chi(coef, req->n_coefficients, 1, ctx->d_primes, chi, ctx->compute));
-        RPS_HIP(rps::launch_resid(ctx->d_primes, chi, 1, nullptr, resid, ctx->compute));
+        RPS_HIP(rps::launch_resid(ctx->d_primes, ctx->d_inverses, chi, 1, nullptr, resid,
+                                  ctx->compute));
 
         // All batches reserve from one flush state, reset once: survivors go
         // to [0, count) of the caller's buffers; a reservation that does not
@@ -1554,7 +1596,7 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
         if (n_build) {
             RPS_HIP(rps::launch_chi(reinterpret_cast<const int *>(dp + plan.coef), n_coef, n_build,
                                     ctx->d_primes, ctx->m_chi.ptr, ctx->compute));
-            RPS_HIP(rps::launch_resid(ctx->d_primes, ctx->m_chi.ptr, n_build,
+            RPS_HIP(rps::launch_resid(ctx->d_primes, ctx->d_inverses, ctx->m_chi.ptr, n_build,
                                       reinterpret_cast<const int *>(dp + plan.slot),
                                       ctx->t_resid.ptr, ctx->compute));
         }

@@ -1510,53 +1510,61 @@ __global__ __launch_bounds__(512) void chi_kernel(
     }
 }
 
-// Residue strings: one wave per (prime index i, u, curve).  For u = 0 (p | b)
-// the string is never read and is stored as zero.  Bit z of the string is
-// chi_p((z mod p) * u^-1 mod p); ballot packs 64 consecutive z per word.
+// Residue strings: one workgroup of 4 waves per (prime index i, chunk of 64
+// u, curve); the curve's chi_p bitmap in LDS, each wave writing the strings
+// of 16 u.  Bit z of string u is chi_p((z mod p) * u^-1 mod p), and ballot
+// packs 64 consecutive z per word; lane l walks z = l + 64 k, whose product
+// with u^-1 advances by 64 u^-1 mod p from one word to the next (z and z - p
+// give the same product mod p, so no reduction of z is needed).  For u = 0
+// (p | b) and u >= p the string is never read and is stored as zero.
+constexpr int kResidChunk = 64;  // u per workgroup
 __global__ __launch_bounds__(256) void resid_kernel(
     const int *__restrict__ primes,
+    const int *__restrict__ inverses,        // [36][512]
     const unsigned long long *__restrict__ chi,
     const int *__restrict__ slot,            // [curves] table slot, or null
     unsigned long long *__restrict__ resid) {  // [slots][36][512][9]
+    __shared__ unsigned long long ch[kChiWords];
     const int i = blockIdx.x;
     const size_t curve = blockIdx.z;
-    const int lane = threadIdx.x & 63;
-    const int u = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const uint32_t p = primes[i];
-    const unsigned long long *ch = chi + (curve * kCandidates + i) * kChiWords;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = static_cast<int>(uniform(static_cast<uint32_t>(tid >> 6)));
+    if (tid < kChiWords) {
+        ch[tid] = chi[(curve * kCandidates + i) * kChiWords + tid];
+    }
+    __syncthreads();
+    const uint32_t p = static_cast<uint32_t>(primes[i]);
+    const uint32_t magic = static_cast<uint32_t>(0x100000000ULL / p);
     const size_t table = slot ? static_cast<size_t>(slot[curve]) : curve;
-    unsigned long long *dst = resid + table * kResidTableWords
-        + (static_cast<size_t>(i) * kMaxPrime + u) * kResidWords;
-    if (u == 0 || u >= static_cast<int>(p)) {
+    unsigned long long *base = resid + table * kResidTableWords
+        + static_cast<size_t>(i) * kMaxPrime * kResidWords;
+    for (int j = wave; j < kResidChunk; j += 4) {
+        const int u = static_cast<int>(blockIdx.y) * kResidChunk + j;
+        unsigned long long *dst = base + static_cast<size_t>(u) * kResidWords;
+        const uint32_t inv = static_cast<uint32_t>(inverses[i * kMaxPrime + u]);
+        if (inv == 0) {  // u = 0 or u >= p
+            if (lane < kResidWords) {
+                dst[lane] = 0;
+            }
+            continue;
+        }
+        const uint32_t step = mod_small(64u * inv, p, magic);
+        uint32_t y = mod_small(static_cast<uint32_t>(lane) * inv, p, magic);
+        unsigned long long mine = 0;
+#pragma unroll
+        for (int k = 0; k < kResidWords; ++k) {
+            const int bit = static_cast<int>((ch[y >> 6] >> (y & 63)) & 1);
+            const unsigned long long word = __ballot(bit);
+            if (lane == k) {
+                mine = word;
+            }
+            const uint32_t next = y + step;
+            y = next >= p ? next - p : next;
+        }
         if (lane < kResidWords) {
-            dst[lane] = 0;
+            dst[lane] = mine;
         }
-        return;
-    }
-    // u^-1 = u^(p-2) mod p (Fermat; p is prime).
-    uint32_t inv = 1, base = u, e = p - 2;
-    while (e) {
-        if (e & 1) {
-            inv = inv * base % p;
-        }
-        base = base * base % p;
-        e >>= 1;
-    }
-    unsigned long long mine = 0;
-    for (int k = 0; k < kResidWords; ++k) {
-        uint32_t z = 64 * k + lane;
-        if (z >= p) {
-            z -= p;  // z < 576 < 2p
-        }
-        const uint32_t y = z * inv % p;
-        const int bit = static_cast<int>((ch[y >> 6] >> (y & 63)) & 1);
-        const unsigned long long word = __ballot(bit);
-        if (lane == k) {
-            mine = word;
-        }
-    }
-    if (lane < kResidWords) {
-        dst[lane] = mine;
     }
 }
 
@@ -1993,11 +2001,12 @@ hipError_t launch_chi(const int *coef_res, int n_coef, int curves,
     return hipGetLastError();
 }
 
-hipError_t launch_resid(const int *primes, const unsigned long long *chi,
-                        int curves, const int *slot, unsigned long long *resid,
-                        hipStream_t stream) {
-    hipLaunchKernelGGL(resid_kernel, dim3(kCandidates, kMaxPrime / 4, curves),
-                       dim3(256), 0, stream, primes, chi, slot, resid);
+hipError_t launch_resid(const int *primes, const int *inverses,
+                        const unsigned long long *chi, int curves, const int *slot,
+                        unsigned long long *resid, hipStream_t stream) {
+    static_assert(kMaxPrime % kResidChunk == 0, "whole chunks of u");
+    hipLaunchKernelGGL(resid_kernel, dim3(kCandidates, kMaxPrime / kResidChunk, curves),
+                       dim3(256), 0, stream, primes, inverses, chi, slot, resid);
     return hipGetLastError();
 }
 

@@ -34,9 +34,11 @@ const VariantInfo *variant(int index);
 hipError_t launch_chi(const int *coef_res, int n_coef, int curves,
                       const int *primes, unsigned long long *chi,
                       hipStream_t stream);
-hipError_t launch_resid(const int *primes, const unsigned long long *chi,
-                        int curves, const int *slot, unsigned long long *resid,
-                        hipStream_t stream);
+// `inverses` is [36][512]: u^-1 mod p for each candidate p and 0 < u < p,
+// 0 elsewhere (the same for every curve, computed once per context).
+hipError_t launch_resid(const int *primes, const int *inverses,
+                        const unsigned long long *chi, int curves, const int *slot,
+                        unsigned long long *resid, hipStream_t stream);
 hipError_t launch_reset(DeviceState *state, hipStream_t stream);
 // Clears a many-curve call's state, tile counter and n failure flags.
 hipError_t launch_reset_many(DeviceState *state, unsigned long long *head, int *failed, int n,
