@@ -381,47 +381,16 @@ struct RowTile {
         }
     }
 
-    // Wide rows: the build as (row, 64-entry column) units, nine or so per
-    // wave, so that every wave of the workgroup takes part (with one thread
-    // per entry only the first kEntries threads had work, and the line
-    // waited for them).  A wave takes consecutive units in row-major order,
-    // so they span at most two rows: the row's constants are read once, and
-    // from one column to the next a lane's string offset advances by 4096
-    // mod p (one modular step) instead of a fresh reduction.  Units are
-    // taken two at a time, their string reads in flight together.
+    // Wide rows: the build as (row, 64-entry column) units dealt to the
+    // waves in turn, so that every wave of the workgroup takes part (with
+    // one thread per entry only the first kEntries threads had work, and
+    // the line waited for them: 10 of 16 waves building 14 entries each);
+    // a unit's row is wave-uniform.  Units are taken two at a time, their
+    // string reads in flight together.
     static constexpr int kCols = kEntries / 64;
     static constexpr int kUnits = FAST * kCols;
     static constexpr int kUnitsPerWave = (kUnits + WAVES - 1) / WAVES;
     static constexpr int kUnitGroup = 2;
-    static_assert(!WIDE || kUnitsPerWave <= kCols, "a wave's units span at most two rows");
-
-    // Row s (wave-uniform) as a wave's build enters it at column `col`.
-    struct BuildRow {
-        uint32_t p, magic, step, o;  // o: this lane's string offset
-        int base, n;                 // first word, element count
-    };
-    __device__ __forceinline__ static BuildRow enter_row(const uint32_t *s_p,
-                                                         const uint32_t *s_c,
-                                                         const uint32_t *s_magic, int s,
-                                                         int col, int lane) {
-        const int sr = s < FAST ? s : 0;
-        BuildRow r;
-        r.p = uniform(s_p[sr]);
-        r.magic = uniform(s_magic[sr]);
-        const uint32_t c = uniform(s_c[sr]);
-        int b = 0, n = 0;
-#pragma unroll
-        for (int k = 0; k < FAST; ++k) {
-            b += k < s ? 2 * slot_elems(k) : 0;
-            n = k == s ? slot_elems(k) : n;
-        }
-        r.base = b;
-        r.n = n;
-        r.step = mod_small(4096u, r.p, r.magic);
-        const uint32_t e = 64u * static_cast<uint32_t>(col) + static_cast<uint32_t>(lane);
-        r.o = mod_small(c + 64u * e, r.p, r.magic);
-        return r;
-    }
 
     __device__ __forceinline__ static void build_rows_dealt(uint64_t *pool, int lane, int wave,
                                                             int par) {
@@ -429,10 +398,6 @@ struct RowTile {
         const uint32_t *s_c = s_p + kSelected;
         const uint32_t *s_magic = s_p + 5 * kSelected;
         const char *cext = reinterpret_cast<const char *>(pool + kCext);
-        const int first = wave * kUnitsPerWave;
-        int s = first / kCols;  // wave-uniform, as is col
-        int col = first - s * kCols;
-        BuildRow r = enter_row(s_p, s_c, s_magic, s, col, lane);
 #pragma unroll
         for (int j0 = 0; j0 < kUnitsPerWave; j0 += kUnitGroup) {
             uint32_t w_lo[kUnitGroup], w_hi[kUnitGroup], e[kUnitGroup], elems[kUnitGroup];
@@ -440,30 +405,31 @@ struct RowTile {
             bool live[kUnitGroup];
 #pragma unroll
             for (int g = 0; g < kUnitGroup; ++g) {
-                live[g] = false;
-                if (j0 + g < kUnitsPerWave) {
-                    if (col == kCols) {
-                        ++s;
-                        col = 0;
-                        r = enter_row(s_p, s_c, s_magic, s, col, lane);
-                    }
-                    live[g] = first + j0 + g < kUnits && s < FAST && 64 * col < r.n + 64;
-                    base[g] = r.base;
-                    elems[g] = static_cast<uint32_t>(r.n);
-                    e[g] = 64u * static_cast<uint32_t>(col) + static_cast<uint32_t>(lane);
-                    const uint32_t o = r.o;
-                    const int sr = s < FAST ? s : 0;
-                    const uint32_t *d =
-                        reinterpret_cast<const uint32_t *>(cext + sr * 80) + (o >> 5);
-                    const uint32_t d0 = d[0];
-                    const uint32_t d1 = d[1];
-                    const uint32_t d2 = d[2];
-                    w_lo[g] = __builtin_amdgcn_alignbit(d1, d0, o & 31);
-                    w_hi[g] = __builtin_amdgcn_alignbit(d2, d1, o & 31);
-                    const uint32_t next = r.o + r.step;
-                    r.o = next >= r.p ? next - r.p : next;
-                    ++col;
+                const int unit = wave + WAVES * (j0 + g);
+                const int s = unit / kCols;
+                const int col = unit - s * kCols;
+                // Row s's base and element count, wave-uniform.
+                int b = 0, n = 0;
+#pragma unroll
+                for (int k = 0; k < FAST; ++k) {
+                    b += k < s ? 2 * slot_elems(k) : 0;
+                    n = k == s ? slot_elems(k) : n;
                 }
+                base[g] = b;
+                elems[g] = static_cast<uint32_t>(n);
+                live[g] = j0 + g < kUnitsPerWave && unit < kUnits && 64 * col < n + 64;
+                const int sr = live[g] ? s : 0;
+                const uint32_t pp = uniform(s_p[sr]);
+                const uint32_t cc = uniform(s_c[sr]);
+                const uint32_t mg = uniform(s_magic[sr]);
+                e[g] = 64u * static_cast<uint32_t>(col) + static_cast<uint32_t>(lane);
+                const uint32_t o = live[g] ? mod_small(cc + 64u * e[g], pp, mg) : 0u;
+                const uint32_t *d = reinterpret_cast<const uint32_t *>(cext + sr * 80) + (o >> 5);
+                const uint32_t d0 = d[0];
+                const uint32_t d1 = d[1];
+                const uint32_t d2 = d[2];
+                w_lo[g] = __builtin_amdgcn_alignbit(d1, d0, o & 31);
+                w_hi[g] = __builtin_amdgcn_alignbit(d2, d1, o & 31);
             }
 #pragma unroll
             for (int g = 0; g < kUnitGroup; ++g) {
