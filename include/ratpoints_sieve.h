@@ -192,7 +192,8 @@ int rps_sieve_many(rps_ctx *ctx, const rps_request *requests,
  *   "min_tiles"          split denominators into word chunks until at least
  *                        this many tiles per launch exist (default 2048)
  *   "direct_words"       lines of at most this many 64-bit words use the
- *                        short-line kernel, 0..2048 (default 1152; 0 = never)
+ *                        short-line kernel, 0..2048 (default 2048; 0 = never;
+ *                        rps_sieve_many takes at most 1152 of it)
  *   "primitive_only"     1: the device drops survivors with gcd(a, b) > 1
  *                        before they are copied (they can never be points;
  *                        the exact stage, src/point_search.cpp:48-60, skips

@@ -303,9 +303,10 @@ struct rps_ctx {
     int variant = -1;  // -1: chosen per request by the tuning table
     int run_log2 = -1;  // -1: the tuning table's (or a forced variant's own) run length
     long long min_tiles = 2048;
-    // Measured crossover of the short-line kernel against the 11-row tile:
-    // 11.3 against 10.4 Tsite/s at 1,024 words, 11.3 against 12.3 at 1,280.
-    unsigned long long direct_words = 1152;
+    // The short-line kernel against the 11-row tile, kernel Tsite/s: 15.3
+    // against 10.4 at 1,024 words, 15.7 against 14.5 at 1,536, 16.6 against
+    // 16.5 at 2,048 (its limit).
+    unsigned long long direct_words = 2048;
     int primitive = 0;  // emit only survivors with gcd(a, b) = 1
     // A kernel wait longer than this fails the call (RPS_EDEVICE) instead
     // of blocking; the context then refuses further work, since its streams
@@ -1349,6 +1350,12 @@ struct SingleSink {
 // tuning table's variant for the request with the most sites.  A single
 // request takes the plain path instead: the same survivors, from a grid
 // and tiles sized for that one curve.
+// The many-curve launch's short-line crossover.  There the row tiles stay
+// ahead of the short-line tiles above about 1,152 words: on the 4,096
+// batched curves, short-line tiles up to 2,048 words made the launches of
+// the 11- and 12-row groups 850 and 488 ms against 812 and 449.
+constexpr unsigned long long kManyDirectWords = 1152;
+
 int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_requests,
                       rps_multi_sink sink, void *user, rps_stats *stats, int forced) {
     if (n_requests == 1 && requests && sink && forced < 0) {
@@ -1417,7 +1424,7 @@ int sieve_many_launch(rps_ctx *ctx, const rps_request *requests, size_t n_reques
             rps::LineDesc L = line_geometry(&requests[r]);
             L.first_t = requests[r].den_first;
             L.n_t = requests[r].den_last - requests[r].den_first + 1;
-            if (L.words <= ctx->direct_words) {
+            if (L.words <= std::min(ctx->direct_words, kManyDirectWords)) {
                 direct_tiling(L, L.n_t);
             } else {
                 const unsigned long long chunks = std::min(

@@ -82,6 +82,6 @@ def fresh_gpu(gpu):
     gpu.set_option("survivor_capacity", 1 << 24)
     gpu.set_option("batch_size", 65536)
     gpu.set_option("min_tiles", 2048)
-    gpu.set_option("direct_words", 1152)
+    gpu.set_option("direct_words", 2048)
     gpu.set_option("primitive_only", 0)
     return gpu

@@ -52,7 +52,7 @@ def variants(ctx):
 def defaults(ctx):
     for name, value in (("variant", -1), ("run_log2", -1), ("survivor_capacity", 1 << 24),
                         ("batch_size", 65536),
-                        ("min_tiles", 2048), ("direct_words", 1152), ("primitive_only", 0)):
+                        ("min_tiles", 2048), ("direct_words", 2048), ("primitive_only", 0)):
         ctx.set_option(name, value)
     return ctx
 

@@ -397,7 +397,7 @@ def test_short_line_kernel_up_to_2048_words(fresh_gpu, oracle_lib, bound):
 
 @pytest.mark.parametrize("bound", [1, 31, 32, 100, 1000, 9000, 24543])
 def test_short_lines_both_modes_match_oracle(fresh_gpu, oracle_lib, bound):
-    # Lines of at most 1,152 words go to the short-line kernel; the same input
+    # Lines of at most 2,048 words go to the short-line kernel; the same input
     # through the row kernel (direct_words = 0) and the oracle must agree.
     coeffs, _ = curves.partially_dense(random.Random(bound), 6, 6)
     first, last = 100, 100 + max(40, 40000 // (2 * bound + 1))
