@@ -497,8 +497,6 @@ rps::LineDesc line_geometry(const rps_request *req) {
     const unsigned long long valid_last = 2ULL * A - 64ULL * (L.words - 1) + 1;
     L.last_mask = valid_last >= 64 ? ~0ULL : ((1ULL << valid_last) - 1);
     L.square = req->square_denominators ? 1 : 0;
-    L.magic_words = L.words == 1 ? 0xFFFFFFFFu
-        : static_cast<unsigned>(0x100000000ULL / std::min<unsigned long long>(L.words, 0xFFFFFFFFULL));
     L.tiles_per_t = 1;
     L.tile_t = 1;
     L.tile_words = L.words;

@@ -63,8 +63,6 @@ struct LineDesc {
     int direct;                          // short lines: direct tiles
     int tiles_per_t;                     // row tiles per denominator
     int tile_t;                          // denominators per tile (a run)
-    unsigned magic_words;                // floor(2^32 / words), direct tiles
-    int pad;
 };
 
 // Where survivors go.
